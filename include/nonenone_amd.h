@@ -1,0 +1,251 @@
+/*
+ * nonenone_amd.h — C ABI of the MI355X per-cell flash + Natural assembly library.
+ *
+ * Every entry point takes plain pointers and sizes.  Array arguments named
+ * `d_*` are DEVICE pointers (HBM, allocated by the caller); `stream` is a
+ * hipStream_t passed as void* (NULL = default stream).  All calls are
+ * asynchronous on `stream` unless stated otherwise.  Return value: NN_OK or a
+ * negative NN_E* code; per-item outcomes are reported in status arrays.
+ *
+ * Units: pressure Pa, temperature K, molar mass kg/mol, critical volume m3/mol.
+ * Phase slots follow the reference: 0 = oil-rich liquid, 1 = gas, 2 = water-rich.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   fluid/EOS configuration   models/include/natural/compositional_mixture.hpp:24-157,
+ *                             models/include/natural/thermo/cubic_eos.hpp:178-311
+ *   flash options             models/include/natural/thermo/three_phase_flash_types.hpp:21-41
+ *   flash / stability         models/include/natural/thermo/three_phase_flash.hpp:83-489
+ *   phase-state update        models/include/natural/state/three_phase_equilibrium.hpp:119-496
+ *   residual / Jacobian       models/include/natural/petsc/detail/natural_petsc_runtime_residual.inc:19-133,
+ *                             models/include/natural/petsc/detail/natural_petsc_runtime_jacobian.inc:8-299
+ *   cached BSR topology       models/include/natural/petsc/jacobian_factory.hpp:27-238
+ *   TPFA connections          grid/include/structuredgrid/structuredgrid.hpp:541-667,877-907
+ */
+#ifndef NONENONE_AMD_H
+#define NONENONE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NN_MAX_NC 8
+
+/* ---- return codes ---------------------------------------------------- */
+#define NN_OK 0
+#define NN_E_INVALID_ARGUMENT (-1)
+#define NN_E_UNSUPPORTED (-2)
+#define NN_E_HIP (-3)
+#define NN_E_NO_DEVICE (-4)
+#define NN_E_OUT_OF_MEMORY (-5)
+
+/* ---- per-item status (uint8/int32 status arrays) ---------------------- */
+#define NN_STATUS_OK 0             /* converged / evaluated                  */
+#define NN_STATUS_NOT_CONVERGED 1  /* flash returned converged=false          */
+#define NN_STATUS_THERMO_ERROR 2   /* reference would throw (no admissible
+                                      root, non-positive volume, ...)         */
+#define NN_STATUS_INVALID_INPUT 3  /* non-finite / non-positive P, T, z       */
+#define NN_STATUS_RECOVERABLE_FAILURE 4 /* phase update restored its inputs   */
+
+/* thermodynamic model, models/include/natural/thermo/thermodynamic_model.hpp:10-15 */
+#define NN_MODEL_PR 0
+#define NN_MODEL_SW 1
+#define NN_MODEL_CPA 2
+
+/*
+ * Fluid description.  Mirrors CompositionalMixture (Tc, Pc, Vc, omega, M, k_ij),
+ * the CubicEquationOfState constructor (Omega_a, Omega_b, eosType, m1, m2),
+ * SoreideWhitsonOptions (water index, salinity, aqueous H2O-solute k_ij) and
+ * CubicPlusAssociationOptions, plus the benchmark Corey relative permeability
+ * (case/h2o_co2_nc10_2d_benchmark/benchmark_common.hpp:97-126).
+ */
+typedef struct nn_fluid_desc {
+    int32_t nc;                 /* number of components, 2..NN_MAX_NC          */
+    int32_t thermo_model;       /* NN_MODEL_*                                  */
+    int32_t eos_type;           /* 1 = PR76 m(omega); 5 = PR78 for omega>0.49  */
+    int32_t water_component;    /* H2O index (three-phase flash needs >= 0)    */
+    double omega_a, omega_b, m1, m2;
+    double tc[NN_MAX_NC], pc[NN_MAX_NC], vc[NN_MAX_NC];
+    double acentric[NN_MAX_NC], molar_mass[NN_MAX_NC];
+    double kij[NN_MAX_NC * NN_MAX_NC];          /* row-major, symmetric       */
+    /* Soreide-Whitson (thermo_model == NN_MODEL_SW) */
+    double sw_salinity;                         /* NaCl molality               */
+    double sw_aqueous_kij[NN_MAX_NC];           /* k(H2O,i) in the Water role  */
+    int32_t sw_aqueous_kij_set[NN_MAX_NC];      /* 0: fall back to kij         */
+    /* CPA (thermo_model == NN_MODEL_CPA) */
+    int32_t cpa_physical_term;                  /* 0 SRK, 1 PR                 */
+    int32_t cpa_max_iterations;                 /* association iterations      */
+    double cpa_tolerance, cpa_damping;
+    double cpa_a0[NN_MAX_NC], cpa_b[NN_MAX_NC], cpa_c1[NN_MAX_NC];
+    double cpa_epsilon[NN_MAX_NC], cpa_beta[NN_MAX_NC];
+    int32_t cpa_donors[NN_MAX_NC], cpa_acceptors[NN_MAX_NC];
+    double cpa_cross_epsilon[NN_MAX_NC * NN_MAX_NC];
+    double cpa_cross_beta[NN_MAX_NC * NN_MAX_NC];
+    /* Corey relative permeability: swc, sorw, sgc, sorg, krw_max, krg_max, kro_max */
+    double relperm[7];
+} nn_fluid_desc;
+
+/* ThreePhaseFlashOptions (three_phase_flash_types.hpp:21-41); waterComponent
+ * is taken from nn_fluid_desc.water_component. */
+typedef struct nn_flash_options {
+    int32_t maximum_iterations;               /* 120 */
+    int32_t maximum_rachford_rice_iterations; /* 80  */
+    int32_t maximum_stability_iterations;     /* 80  */
+    int32_t stability_ssi_before_newton;      /* 8   */
+    int32_t maximum_stability_newton;         /* 12  */
+    int32_t two_phase_ssi_before_newton;      /* 8   */
+    int32_t maximum_two_phase_newton;         /* 12  */
+    int32_t three_phase_ssi_before_newton;    /* 10  */
+    int32_t maximum_three_phase_newton;       /* 12  */
+    double fugacity_tolerance;                /* 1e-9  */
+    double rachford_rice_tolerance;           /* 1e-12 */
+    double stability_tolerance;               /* 1e-9  */
+    double phase_fraction_tolerance;          /* 1e-10 */
+    double coincident_composition_tolerance;  /* 1e-8  */
+    double composition_floor;                 /* 1e-30 */
+    double log_k_step_limit;                  /* 2     */
+    double numerical_jacobian_step;           /* 1e-6  */
+} nn_flash_options;
+
+/* Fills `o` with the reference defaults listed above. */
+void nn_flash_options_default(nn_flash_options *o);
+
+typedef struct nn_context nn_context;
+
+/* Creates a context on the current HIP device: uploads the fluid to device
+ * constant storage.  Replaces constructing CubicEquationOfState +
+ * CubicThreePhaseFlash (three_phase_flash.hpp:47-81).  Synchronous. */
+int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts,
+                      nn_context **out);
+int nn_context_destroy(nn_context *ctx);
+/* Last error text of this context (static storage, never NULL). */
+const char *nn_last_error(void);
+
+/*
+ * E1-E5: batched single-phase EOS evaluation, CubicEquationOfState::phaseResult
+ * with a phase role (cubic_eos.hpp:834-856): mixingParameters -> phaseMixing ->
+ * liquid/vapour root -> fugacityAtRoot.  role[i] in {0 oil,1 gas,2 water}.
+ * Layout: d_x, d_lnphi, d_fug are n x nc row-major.
+ */
+int nn_eos_phase_batch(nn_context *ctx, int64_t n,
+                       const double *d_P, const double *d_T, const double *d_x,
+                       const int32_t *d_role,
+                       double *d_Z, double *d_lnphi, double *d_fug,
+                       int32_t *d_status, void *stream);
+
+/* Flash outputs (ThreePhaseFlashResult, three_phase_flash_types.hpp:43-58),
+ * structure of arrays, all device pointers, n items. */
+typedef struct nn_flash_out {
+    uint8_t *presence;   /* n     : bit0 oil, bit1 gas, bit2 water            */
+    uint8_t *converged;  /* n                                                  */
+    int32_t *status;     /* n     : NN_STATUS_*                               */
+    int32_t *iterations; /* n                                                  */
+    double *beta;        /* n x 3 : phase mole fractions                      */
+    double *saturation;  /* n x 3                                             */
+    double *x;           /* n x 3 x nc : phase compositions                   */
+    double *Z;           /* n x 3 : compressibility factors                   */
+    double *rho_m;       /* n x 3 : molar densities                           */
+    double *k_go;        /* n x nc: vaporOilK                                 */
+    double *k_wo;        /* n x nc: waterOilK                                 */
+} nn_flash_out;
+
+/* F2: CubicThreePhaseFlash::flash(P,T,z) (three_phase_flash.hpp:83-227) on n
+ * independent states.  d_z is n x nc.  Persistent work-stealing launch. */
+int nn_flash_batch(nn_context *ctx, int64_t n,
+                   const double *d_P, const double *d_T, const double *d_z,
+                   const nn_flash_out *out, void *stream);
+
+/* F-restricted: CubicThreePhaseFlash::flashRestricted(P,T,z,allowed[,seed])
+ * (three_phase_flash.hpp:235-292).  d_allowed: n presence masks; d_seed may be
+ * NULL (no composition seed) or n x 3 x nc. */
+int nn_flash_restricted_batch(nn_context *ctx, int64_t n,
+                              const double *d_P, const double *d_T,
+                              const double *d_z, const uint8_t *d_allowed,
+                              const double *d_seed,
+                              const nn_flash_out *out, void *stream);
+
+/* F7: CubicThreePhaseFlash::stabilityTest(P,T,z,active,x) (three_phase_flash.hpp:300-489).
+ * d_x: n x 3 x nc active compositions.  Outputs: stable/valid flags,
+ * missing-phase-unstable mask (bit per slot), trial sums n x 3 and incipient
+ * compositions n x 3 x nc. */
+int nn_stability_batch(nn_context *ctx, int64_t n,
+                       const double *d_P, const double *d_T, const double *d_z,
+                       const uint8_t *d_active, const double *d_x,
+                       uint8_t *d_stable, uint8_t *d_valid,
+                       uint8_t *d_unstable_mask, double *d_trial_sum,
+                       double *d_incipient, int32_t *d_status, void *stream);
+
+/*
+ * S1: FullyCompositionalThreePhaseEquilibrium::updatePhaseState
+ * (three_phase_equilibrium.hpp:282-496) on n cells, in place.
+ * d_primary: n x np (np = 3nc+1, +1 with a well unknown), reference Primary layout
+ *            p, x_o[0..nc-2], x_g[..], x_w[..], S_o, S_g, S_w [, p_bhp].
+ * d_phase_state: n x (3nc+8), reference PhaseState Vec layout
+ *            flag, K_go[nc], K_wo[nc], z[nc], beta[3], Z_o, Z_g, Z_w, suppression.
+ * d_T: per-cell temperature.  d_status: NN_STATUS_* (RECOVERABLE_FAILURE =
+ * inputs restored).  d_flags: bit0 phaseRemoved, bit1 stabilityInvalid,
+ * bit2 missingPhaseUnstable, bit3 restrictedFlashFailed,
+ * bit4 unrestrictedFlashFailed, bit5 stableReducedSet (may be NULL).
+ */
+int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown,
+                          const double *d_T, double *d_primary,
+                          double *d_phase_state, int32_t *d_status,
+                          uint8_t *d_flags, void *stream);
+
+/* ---- Natural assembly -------------------------------------------------- */
+
+/* Host-side setup (no device work): 7-point structured TPFA connections in the
+ * reference face order X-,X+,Y-,Y+,Z-,Z+ with harmonic transmissibility and
+ * g*dz on Z faces (structuredgrid.hpp:541-667,877-907), plus the cached BSR
+ * pattern: per row block the sorted unique column blocks {self, neighbours}
+ * (jacobian_factory.hpp:187-201).  perm is ncell x 3 (kx,ky,kz), dx/dy/dz are
+ * per-axis widths.  Output sizes: conn_offsets ncell+1, conn_* nconn
+ * (<= 6*ncell), row_offsets ncell+1, col_blocks nnzb.  Pass NULL outputs to
+ * query counts via *nconn / *nnzb.  Cell order: i fastest, then j, then k. */
+int nn_structured_topology(int32_t nx, int32_t ny, int32_t nz,
+                           const double *dx, const double *dy, const double *dz,
+                           const double *perm,
+                           int64_t *conn_offsets, int64_t *conn_neighbor,
+                           double *conn_trans, double *conn_gdz,
+                           int64_t *row_offsets, int64_t *col_blocks,
+                           int64_t *nconn, int64_t *nnzb);
+
+typedef struct nn_assembly nn_assembly;
+
+/* Uploads topology (device pointers, kept by reference, must outlive the
+ * handle) and precomputes, per connection, the BSR slot of its reciprocal block
+ * and, per cell, the slot of its diagonal block.  Synchronous. */
+int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
+                       const int64_t *d_conn_offsets, const int64_t *d_conn_neighbor,
+                       const double *d_conn_trans, const double *d_conn_gdz,
+                       const int64_t *d_row_offsets, const int64_t *d_col_blocks,
+                       int64_t nnzb, const double *d_volume,
+                       const double *d_porosity, const double *d_T,
+                       nn_assembly **out);
+int nn_assembly_destroy(nn_assembly *a);
+
+/* R6: accepted-step accumulation M^n (per cell nc doubles) from a state
+ * (natural_petsc_runtime_cache_history.inc:177-260). */
+int nn_assembly_accumulation(nn_assembly *a, const double *d_primary,
+                             const double *d_phase_state, double *d_accum,
+                             void *stream);
+
+/*
+ * R2 + R3: residual F (ncell x np) and BSR Jacobian values (nnzb x np x np,
+ * row-major blocks, in the cached pattern's rowOffsets/columnBlocks order),
+ * exactly the values the reference writes with MatSetValuesBlocked
+ * (scaling disabled).  d_residual or d_bsr may be NULL to skip that part.
+ * d_decisions (ncell... nconn x 3 uint8 upwind flags, may be NULL).
+ */
+int nn_assembly_evaluate(nn_assembly *a, double dt, double fugacity_scaling,
+                         const double *d_primary, const double *d_phase_state,
+                         const double *d_accum_prev, double *d_residual,
+                         double *d_bsr, uint8_t *d_decisions,
+                         int32_t *d_status, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NONENONE_AMD_H */

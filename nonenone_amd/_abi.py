@@ -1,0 +1,181 @@
+"""ctypes mirror of include/nonenone_amd.h and the loader of the HIP library.
+
+The product library is ``nonenone_amd/libnonenone_amd.so`` (built in-tree by
+``nonenone_amd/build.py``).  There is no CPU fallback: :func:`lib` raises when
+the library is missing, and every compute entry point launches HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+NN_MAX_NC = 8
+
+NN_OK = 0
+NN_STATUS_OK = 0
+NN_STATUS_NOT_CONVERGED = 1
+NN_STATUS_THERMO_ERROR = 2
+NN_STATUS_INVALID_INPUT = 3
+NN_STATUS_RECOVERABLE_FAILURE = 4
+
+NN_MODEL_PR = 0
+NN_MODEL_SW = 1
+NN_MODEL_CPA = 2
+
+_D = ctypes.c_double
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_P = ctypes.c_void_p
+
+
+class FluidDesc(ctypes.Structure):
+    """nn_fluid_desc."""
+
+    _fields_ = [
+        ("nc", _I32),
+        ("thermo_model", _I32),
+        ("eos_type", _I32),
+        ("water_component", _I32),
+        ("omega_a", _D),
+        ("omega_b", _D),
+        ("m1", _D),
+        ("m2", _D),
+        ("tc", _D * NN_MAX_NC),
+        ("pc", _D * NN_MAX_NC),
+        ("vc", _D * NN_MAX_NC),
+        ("acentric", _D * NN_MAX_NC),
+        ("molar_mass", _D * NN_MAX_NC),
+        ("kij", _D * (NN_MAX_NC * NN_MAX_NC)),
+        ("sw_salinity", _D),
+        ("sw_aqueous_kij", _D * NN_MAX_NC),
+        ("sw_aqueous_kij_set", _I32 * NN_MAX_NC),
+        ("cpa_physical_term", _I32),
+        ("cpa_max_iterations", _I32),
+        ("cpa_tolerance", _D),
+        ("cpa_damping", _D),
+        ("cpa_a0", _D * NN_MAX_NC),
+        ("cpa_b", _D * NN_MAX_NC),
+        ("cpa_c1", _D * NN_MAX_NC),
+        ("cpa_epsilon", _D * NN_MAX_NC),
+        ("cpa_beta", _D * NN_MAX_NC),
+        ("cpa_donors", _I32 * NN_MAX_NC),
+        ("cpa_acceptors", _I32 * NN_MAX_NC),
+        ("cpa_cross_epsilon", _D * (NN_MAX_NC * NN_MAX_NC)),
+        ("cpa_cross_beta", _D * (NN_MAX_NC * NN_MAX_NC)),
+        ("relperm", _D * 7),
+    ]
+
+
+class FlashOptions(ctypes.Structure):
+    """nn_flash_options (ThreePhaseFlashOptions)."""
+
+    _fields_ = [
+        ("maximum_iterations", _I32),
+        ("maximum_rachford_rice_iterations", _I32),
+        ("maximum_stability_iterations", _I32),
+        ("stability_ssi_before_newton", _I32),
+        ("maximum_stability_newton", _I32),
+        ("two_phase_ssi_before_newton", _I32),
+        ("maximum_two_phase_newton", _I32),
+        ("three_phase_ssi_before_newton", _I32),
+        ("maximum_three_phase_newton", _I32),
+        ("fugacity_tolerance", _D),
+        ("rachford_rice_tolerance", _D),
+        ("stability_tolerance", _D),
+        ("phase_fraction_tolerance", _D),
+        ("coincident_composition_tolerance", _D),
+        ("composition_floor", _D),
+        ("log_k_step_limit", _D),
+        ("numerical_jacobian_step", _D),
+    ]
+
+    @classmethod
+    def defaults(cls) -> "FlashOptions":
+        """Reference defaults, three_phase_flash_types.hpp:21-41."""
+        return cls(120, 80, 80, 8, 12, 8, 12, 10, 12,
+                   1e-9, 1e-12, 1e-9, 1e-10, 1e-8, 1e-30, 2.0, 1e-6)
+
+
+class FlashOut(ctypes.Structure):
+    """nn_flash_out: device pointers."""
+
+    _fields_ = [
+        ("presence", _P),
+        ("converged", _P),
+        ("status", _P),
+        ("iterations", _P),
+        ("beta", _P),
+        ("saturation", _P),
+        ("x", _P),
+        ("Z", _P),
+        ("rho_m", _P),
+        ("k_go", _P),
+        ("k_wo", _P),
+    ]
+
+
+_PRODUCT_LIB = None
+
+
+def library_path() -> Path:
+    return Path(__file__).resolve().parent / "libnonenone_amd.so"
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library.  Raises when it is not built: no CPU fallback."""
+    global _PRODUCT_LIB
+    if _PRODUCT_LIB is not None:
+        return _PRODUCT_LIB
+    path = library_path()
+    if not path.exists():
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            " (hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = ctypes.CDLL(str(path), mode=os.RTLD_GLOBAL if hasattr(os, "RTLD_GLOBAL") else 0)
+    _declare(L)
+    _PRODUCT_LIB = L
+    return L
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    L.nn_flash_options_default.argtypes = [ctypes.POINTER(FlashOptions)]
+    L.nn_flash_options_default.restype = None
+    L.nn_context_create.argtypes = [ctypes.POINTER(FluidDesc), ctypes.POINTER(FlashOptions),
+                                    ctypes.POINTER(_P)]
+    L.nn_context_destroy.argtypes = [_P]
+    L.nn_last_error.restype = ctypes.c_char_p
+    L.nn_eos_phase_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.nn_flash_batch.argtypes = [_P, _I64, _P, _P, _P, ctypes.POINTER(FlashOut), _P]
+    L.nn_flash_restricted_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P,
+                                            ctypes.POINTER(FlashOut), _P]
+    L.nn_stability_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.nn_phase_update_batch.argtypes = [_P, _I64, _I32, _P, _P, _P, _P, _P, _P]
+    L.nn_structured_topology.argtypes = [_I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P,
+                                         _P, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]
+    L.nn_assembly_create.argtypes = [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P,
+                                     ctypes.POINTER(_P)]
+    L.nn_assembly_destroy.argtypes = [_P]
+    L.nn_assembly_accumulation.argtypes = [_P, _P, _P, _P, _P]
+    L.nn_assembly_evaluate.argtypes = [_P, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P]
+    for name in ("nn_context_create", "nn_context_destroy", "nn_eos_phase_batch",
+                 "nn_flash_batch", "nn_flash_restricted_batch", "nn_stability_batch",
+                 "nn_phase_update_batch", "nn_structured_topology", "nn_assembly_create",
+                 "nn_assembly_destroy", "nn_assembly_accumulation", "nn_assembly_evaluate"):
+        getattr(L, name).restype = ctypes.c_int
+
+
+# Every symbol include/nonenone_amd.h declares.
+EXPORTED_SYMBOLS = (
+    "nn_flash_options_default", "nn_context_create", "nn_context_destroy", "nn_last_error",
+    "nn_eos_phase_batch", "nn_flash_batch", "nn_flash_restricted_batch",
+    "nn_stability_batch", "nn_phase_update_batch", "nn_structured_topology",
+    "nn_assembly_create", "nn_assembly_destroy", "nn_assembly_accumulation",
+    "nn_assembly_evaluate",
+)
+
+
+def check(rc: int, what: str) -> None:
+    if rc != NN_OK:
+        msg = lib().nn_last_error()
+        raise RuntimeError(f"{what} failed with code {rc}: {msg.decode() if msg else ''}")

@@ -1,0 +1,78 @@
+"""Build libnonenone_amd.so for gfx950 with hipcc (in-tree, no JIT cache).
+
+    python -m nonenone_amd.build          # or __graft_entry__.build()
+
+The kernels for each component count live in their own translation unit
+(csrc/nn_nc<NC>.hip) and compile in parallel; csrc/nn_api.hip holds the C ABI.
+
+Flags: -ffp-contract=off keeps +,-,*,/ rounding identical to the reference's
+host build (no fused multiply-add), so the flash follows the same iterates.
+-Xclang -target-feature -Xclang -packed-fp32-ops is required on these
+machines (packed FP32 instructions are disabled).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+OUT = HERE / "libnonenone_amd.so"
+BUILD_DIR = HERE.parent / "_build"
+ARCH = os.environ.get("NN_OFFLOAD_ARCH", "gfx950")
+UNITS = ("nn_api.hip", "nn_nc3.hip", "nn_nc4.hip", "nn_nc6.hip")
+FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+
+
+def hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (ROCm toolchain required)")
+
+
+def _filtered(stderr: str) -> str:
+    # hipcc's host pass warns about the device-only target feature; that is harmless.
+    return "\n".join(line for line in stderr.splitlines()
+                     if "packed-fp32-ops" not in line and line.strip())
+
+
+def _compile(unit: str) -> tuple[str, Path, str]:
+    obj = BUILD_DIR / (unit + ".o")
+    cmd = [hipcc(), *FLAGS, "-I", str(HERE.parent / "include"), "-c", str(CSRC / unit),
+           "-o", str(obj)]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {unit} ({proc.returncode}):\n{_filtered(proc.stderr)}")
+    return unit, obj, _filtered(proc.stderr)
+
+
+def build(verbose: bool = True) -> Path:
+    BUILD_DIR.mkdir(exist_ok=True)
+    t0 = time.time()
+    with ThreadPoolExecutor(max_workers=len(UNITS)) as pool:
+        results = list(pool.map(_compile, UNITS))
+    tmp = OUT.with_suffix(".so.tmp")
+    link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC",
+            *(str(o) for _, o, _ in results), "-o", str(tmp)]
+    proc = subprocess.run(link, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"link failed:\n{_filtered(proc.stderr)}")
+    os.replace(tmp, OUT)
+    if verbose:
+        for unit, _, warn in results:
+            if warn:
+                print(f"[{unit}]\n{warn}", file=sys.stderr)
+        print(f"built {OUT} in {time.time() - t0:.1f}s", file=sys.stderr)
+    return OUT
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,90 @@
+// common.hpp — device-side fluid parameters, options and small helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nonenone_amd.h"
+
+namespace nn
+{
+
+constexpr int kMaxNc = NN_MAX_NC;
+constexpr double kGasConstant = 8.31446261815324;     // common/include/common/units.hpp
+constexpr double kBar = 1.0e5;
+// Unit constants evaluated with the same expressions as common/units.hpp:19-53
+// so that they round identically.
+constexpr double kPsi = (0.45359237 * 9.80665) / (0.0254 * 0.0254);
+constexpr double kKelvinPerRankine = 5.0 / 9.0;
+constexpr double kCentipoise = 1.0e-2 * 0.1;
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kDblMax = 1.7976931348623157e308;
+constexpr double kLogDblMax = 709.782712893384;
+
+// Phase slots / thermodynamic roles (phase_behavior.hpp:17-22).
+constexpr int kOil = 0;
+constexpr int kGas = 1;
+constexpr int kWater = 2;
+constexpr uint8_t kOilBit = 1, kGasBit = 2, kWaterBit = 4, kAllBits = 7;
+
+// Error codes raised where the reference throws; they abort the enclosing
+// public call exactly as the exception would.
+enum : int
+{
+    kErrNone = 0,
+    kErrThermo = NN_STATUS_THERMO_ERROR,
+    kErrInvalid = NN_STATUS_INVALID_INPUT,
+};
+
+// Fluid parameters resident in HBM; kernels read them through uniform loads.
+struct FluidParams
+{
+    int nc;
+    int model;        // NN_MODEL_*
+    int eosType;      // 1 or 5
+    int water;        // H2O component (-1 if none)
+    double omegaA, omegaB, m1, m2;
+    double tc[kMaxNc], pc[kMaxNc], vc[kMaxNc], omega[kMaxNc], mw[kMaxNc];
+    double kij[kMaxNc][kMaxNc];
+    double swSalinity;
+    double swAqKij[kMaxNc];
+    int swAqSet[kMaxNc];
+    // LBC constants (compositional_properties.hpp:166-182), computed on the host
+    // with the reference's expression order.
+    double lbcMwSqrt[kMaxNc];
+    double lbcEps[kMaxNc];
+    double relperm[7];
+    // CPA
+    int cpaTerm, cpaMaxIter;
+    double cpaTol, cpaDamping;
+    double cpaA0[kMaxNc], cpaB[kMaxNc], cpaC1[kMaxNc], cpaEps[kMaxNc], cpaBeta[kMaxNc];
+    int cpaDonors[kMaxNc], cpaAcceptors[kMaxNc];
+    double cpaCrossEps[kMaxNc][kMaxNc], cpaCrossBeta[kMaxNc][kMaxNc];
+};
+
+struct FlashOptions
+{
+    int maxIterations, maxRrIterations, maxStabilityIterations;
+    int stabilitySsi, maxStabilityNewton, twoPhaseSsi, maxTwoPhaseNewton;
+    int threePhaseSsi, maxThreePhaseNewton;
+    double fugTol, rrTol, stabTol, betaTol, coincidentTol, floor, logKStep, fdStep;
+};
+
+__host__ __device__ inline double clampd(double v, double lo, double hi)
+{
+    return v < lo ? lo : (hi < v ? hi : v);
+}
+__host__ __device__ inline double maxd(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ inline double mind(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ inline int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (hi < v ? hi : v);
+}
+__host__ __device__ inline bool finite(double v) { return __builtin_isfinite(v); }
+
+__host__ __device__ inline int popcount3(uint8_t b)
+{
+    return (b & 1) + ((b >> 1) & 1) + ((b >> 2) & 1);
+}
+
+} // namespace nn

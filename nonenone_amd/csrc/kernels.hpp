@@ -1,0 +1,465 @@
+// kernels.hpp — HIP kernels of the library and the Launch<NC> definitions.
+// Included once per component count by nn_nc<NC>.hip.
+#pragma once
+
+#include "assembly.hpp"
+#include "common.hpp"
+#include "eos.hpp"
+#include "flash.hpp"
+#include "launch.hpp"
+#include "phase_update.hpp"
+
+namespace nn
+{
+
+// ------------------------------------------------------------ E1-E5 kernel
+template <int NC>
+__global__ void __launch_bounds__(kBlock) eosPhaseKernel(
+    const FluidParams *__restrict__ fp, int64_t n, const double *__restrict__ P,
+    const double *__restrict__ T, const double *__restrict__ x, const int32_t *__restrict__ role,
+    double *__restrict__ Z, double *__restrict__ lnphi, double *__restrict__ fug,
+    int32_t *__restrict__ status)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const FluidParams &f = *fp;
+    Err err;
+    double xs[NC];
+    for (int c = 0; c < NC; ++c) xs[c] = x[s * NC + c];
+    const int r = role[s];
+    MixParams<NC, double> mp;
+    mixingParameters<NC, double>(f, P[s], T[s], r, mp);
+    PhaseMix<NC, double> mix;
+    phaseMixing<NC, double>(xs, mp, mix);
+    const double z = selectRoot(f, mix.A, mix.B, r != kGas, err);
+    double lp[NC];
+    if (!err.bad())
+        lnPhiAtRoot<NC, double>(f, mp, mix, z, lp, err);
+    Z[s] = z;
+    for (int c = 0; c < NC; ++c)
+    {
+        const double phi = err.bad() ? 0.0 : ::exp(lp[c]);
+        lnphi[s * NC + c] = err.bad() ? 0.0 : ::log(phi);
+        fug[s * NC + c] = phi * P[s] * xs[c];
+    }
+    status[s] = err.code;
+}
+
+// ---------------------------------------------------------- flash kernels
+template <int NC>
+__device__ void storeFlash(const FlashResult<NC> &r, int64_t s, const nn_flash_out &out)
+{
+    out.presence[s] = r.presence;
+    out.converged[s] = r.converged ? 1 : 0;
+    out.iterations[s] = r.iterations;
+    for (int p = 0; p < 3; ++p)
+    {
+        out.beta[s * 3 + p] = r.beta[p];
+        out.saturation[s * 3 + p] = r.sat[p];
+        out.Z[s * 3 + p] = r.Z[p];
+        out.rho_m[s * 3 + p] = r.rho[p];
+        for (int c = 0; c < NC; ++c)
+            out.x[(s * 3 + p) * NC + c] = r.x[p][c];
+    }
+    for (int c = 0; c < NC; ++c)
+    {
+        out.k_go[s * NC + c] = r.kg[c];
+        out.k_wo[s * NC + c] = r.kw[c];
+    }
+}
+
+template <int NC>
+__device__ void flashOne(const FluidParams &f, const FlashOptions &o, int64_t s,
+                         const double *P, const double *T, const double *z,
+                         const uint8_t *allowed, const double *seed, const nn_flash_out &out)
+{
+    Err err;
+    Flash<NC> fl(f, o, err);
+    double zs[NC];
+    for (int c = 0; c < NC; ++c) zs[c] = z[s * NC + c];
+    FlashResult<NC> r;
+    if (allowed == nullptr)
+    {
+        r = fl.flash(P[s], T[s], zs);
+    }
+    else
+    {
+        const uint8_t a = allowed[s];
+        if (a == kAllBits)
+        {
+            // flashRestricted with every phase allowed: the full flash followed
+            // by the restricted post-processing (three_phase_flash.hpp:235-259).
+            double zn[NC];
+            for (int c = 0; c < NC; ++c) zn[c] = zs[c];
+            fl.normalize(zn);
+            if (!err.bad())
+                r = fl.flash(P[s], T[s], zn);
+            if (!err.bad() && r.converged)
+            {
+                fl.canonicalizeFinal(P[s], T[s], r, true);
+                if (!err.bad() && r.converged && !fl.validatePhysical(P[s], T[s], zn, r))
+                    r.converged = false;
+            }
+        }
+        else if (seed == nullptr)
+        {
+            r = fl.flashRestricted(P[s], T[s], zs, a, nullptr);
+        }
+        else
+        {
+            double sd[3][NC];
+            for (int p = 0; p < 3; ++p)
+                for (int c = 0; c < NC; ++c) sd[p][c] = seed[(s * 3 + p) * NC + c];
+            r = fl.flashRestricted(P[s], T[s], zs, a, sd);
+        }
+    }
+    storeFlash<NC>(r, s, out);
+    int st = err.code;
+    if (st == kErrNone)
+        st = r.converged ? NN_STATUS_OK : NN_STATUS_NOT_CONVERGED;
+    if (err.bad())
+        out.converged[s] = 0;
+    out.status[s] = st;
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kBlock) flashKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
+    const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
+    const uint8_t *__restrict__ allowed, const double *__restrict__ seed, nn_flash_out out,
+    unsigned int *__restrict__ counter)
+{
+    // Persistent work queue: each work-item claims states until none are left.
+    for (;;)
+    {
+        const int64_t s = int64_t(atomicAdd(counter, 1u));
+        if (s >= n)
+            break;
+        flashOne<NC>(*fp, *op, s, P, T, z, allowed, seed, out);
+    }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kBlock) stabilityKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
+    const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
+    const uint8_t *__restrict__ active, const double *__restrict__ xin, uint8_t *stable,
+    uint8_t *valid, uint8_t *mask, double *trial, double *incipient, int32_t *status)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    Err err;
+    Flash<NC> fl(*fp, *op, err);
+    double zs[NC], xs[3][NC];
+    for (int c = 0; c < NC; ++c) zs[c] = z[s * NC + c];
+    for (int p = 0; p < 3; ++p)
+        for (int c = 0; c < NC; ++c) xs[p][c] = xin[(s * 3 + p) * NC + c];
+    const StabilityResult<NC> r = fl.stabilityTest(P[s], T[s], zs, active[s], xs);
+    stable[s] = r.stable;
+    valid[s] = r.valid;
+    uint8_t m = 0;
+    for (int p = 0; p < 3; ++p)
+    {
+        if (r.unstable[p]) m |= uint8_t(1u << p);
+        trial[s * 3 + p] = r.trialSum[p];
+        for (int c = 0; c < NC; ++c) incipient[(s * 3 + p) * NC + c] = r.incipient[p][c];
+    }
+    mask[s] = m;
+    status[s] = err.code;
+}
+
+// ---------------------------------------------------- phase-update kernel
+template <int NC, bool Well>
+__global__ void __launch_bounds__(kBlock) phaseUpdateKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
+    const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
+    int32_t *__restrict__ status, uint8_t *__restrict__ flags, unsigned int *__restrict__ counter)
+{
+    for (;;)
+    {
+        const int64_t s = int64_t(atomicAdd(counter, 1u));
+        if (s >= n)
+            break;
+        phaseUpdateCell<NC, Well>(*fp, *op, T[s], primary, phaseState, s, status, flags);
+    }
+}
+
+// ------------------------------------------------------- assembly kernels
+
+
+template <int NC, bool Well>
+__device__ bool loadCell(const FluidParams &f, const AssemblyView &v, const double *primary,
+                         const double *phaseState, int64_t c, uint8_t &presence)
+{
+    using L = Layout<NC, Well>;
+    const double flag = phaseState[c * L::NPS + L::flag];
+    const int raw = int(flag + (flag >= 0.0 ? 0.5 : -0.5));
+    if (raw <= 0 || raw > 7)
+        return false;
+    presence = uint8_t(raw);
+    return true;
+}
+
+// Value-only property pass: the record a neighbour's face flux reads.
+template <int NC, bool Well>
+__global__ void __launch_bounds__(256) cellRecordKernel(
+    const FluidParams *__restrict__ fp, AssemblyView v, const double *__restrict__ primary,
+    const double *__restrict__ phaseState, double *__restrict__ accum, int32_t *__restrict__ status)
+{
+    using L = Layout<NC, Well>;
+    using S = Dual<0>;
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= v.ncell)
+        return;
+    const FluidParams &f = *fp;
+    uint8_t presence = 0;
+    Err err;
+    CellStateT<NC, S> st;
+    CellPropsT<NC, S> pr;
+    if (!loadCell<NC, Well>(f, v, primary, phaseState, c, presence))
+        err.raise(kErrThermo);
+    else
+    {
+        decodeState<NC, Well, S>(primary + c * L::NP, presence, st);
+        evaluateProperties<NC, S>(f, st, v.porosity[c], v.T[c], pr, err);
+    }
+    if (status)
+        status[c] = err.code;
+    const double nan = __builtin_nan("");
+    const int64_t n = v.ncell;
+    if (v.record)
+    {
+        double *r = v.record;
+        const bool ok = !err.bad();
+        r[0 * n + c] = ok ? st.p.v : nan;
+        for (int q = 0; q < 3; ++q)
+        {
+            r[(1 + q) * n + c] = ok ? pr.rho[q].v : nan;
+            r[(4 + q) * n + c] = ok ? pr.sat[q].v : nan;
+            r[(7 + q) * n + c] = ok ? pr.mob[q].v : nan;
+            for (int k = 0; k < NC; ++k)
+                r[(10 + q * NC + k) * n + c] = ok ? pr.w[q][k].v : nan;
+        }
+    }
+    if (accum)
+        for (int k = 0; k < NC; ++k)
+            accum[c * NC + k] = err.bad() ? nan : componentAccumulation<NC, S>(pr, k).v;
+}
+
+// Residual + Jacobian column of one cell (residual.inc:59-120, jacobian.inc:20-237).
+template <int NC, bool Well>
+__global__ void __launch_bounds__(64) assembleKernel(
+    const FluidParams *__restrict__ fp, AssemblyView v, double dt, double fugScale,
+    const double *__restrict__ primary, const double *__restrict__ phaseState,
+    const double *__restrict__ accumPrev, double *__restrict__ residual, double *__restrict__ bsr,
+    uint8_t *__restrict__ decisions, int32_t *__restrict__ status)
+{
+    using L = Layout<NC, Well>;
+    constexpr int NP = L::NP;
+    using S = Dual<NP>;
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= v.ncell)
+        return;
+    const FluidParams &f = *fp;
+    Err err;
+    uint8_t presence = 0;
+    CellStateT<NC, S> st;
+    CellPropsT<NC, S> pr;
+    if (!loadCell<NC, Well>(f, v, primary, phaseState, c, presence))
+        err.raise(kErrThermo);
+    else
+    {
+        decodeState<NC, Well, S>(primary + c * NP, presence, st);
+        evaluateProperties<NC, S>(f, st, v.porosity[c], v.T[c], pr, err);
+    }
+    const double nan = __builtin_nan("");
+    if (err.bad())
+    {
+        if (status) status[c] = err.code;
+        if (residual)
+            for (int r = 0; r < NP; ++r) residual[c * NP + r] = nan;
+        if (bsr)
+        {
+            double *self = bsr + v.selfSlot[c] * NP * NP;
+            for (int e = 0; e < NP * NP; ++e) self[e] = nan;
+            for (int64_t k = v.connOffsets[c]; k < v.connOffsets[c + 1]; ++k)
+            {
+                double *rb = bsr + v.recipSlot[k] * NP * NP;
+                for (int e = 0; e < NP * NP; ++e) rb[e] = nan;
+            }
+        }
+        return;
+    }
+    S res[NP];
+    localResidual<NC, Well, S>(st, pr, accumPrev + c * NC, v.volume[c], dt, fugScale, res);
+
+    double *self = bsr ? bsr + v.selfSlot[c] * NP * NP : nullptr;
+    if (self)
+        for (int r = 0; r < NP; ++r)
+            for (int q = 0; q < NP; ++q) self[r * NP + q] = res[r].d[q];
+
+    double outward[NC];
+    for (int k = 0; k < NC; ++k) outward[k] = 0.0;
+    const int64_t n = v.ncell;
+    const double *rec = v.record;
+    const double vIn = v.volume[c];
+    for (int64_t k = v.connOffsets[c]; k < v.connOffsets[c + 1]; ++k)
+    {
+        const int64_t nb = v.connNeighbor[k];
+        const double trans = v.connTrans[k];
+        const double gdz = v.connGdz[k];
+        const double pEx = rec[nb];
+        const double vEx = v.volume[nb];
+        bool upIn[3];
+        double faceFlux[NC];
+        for (int i = 0; i < NC; ++i) faceFlux[i] = 0.0;
+        // scalar face flux (face_flux.hpp:330-430)
+        for (int q = 0; q < 3; ++q)
+        {
+            const double rhoIn = pr.rho[q].v, rhoEx = rec[(1 + q) * n + nb];
+            const double satIn = pr.sat[q].v, satEx = rec[(4 + q) * n + nb];
+            const double avg = 0.5 * (rhoIn * satIn + rhoEx * satEx) /
+                               maxd(0.5 * (satIn + satEx), 1.0e-8);
+            const double dPhi = pEx + avg * gdz - st.p.v;
+            upIn[q] = interiorUpwind(dPhi, vIn, vEx, c, nb);
+            if (decisions)
+                decisions[k * 3 + q] = upIn[q] ? 0 : 1;
+            const double mob = upIn[q] ? pr.mob[q].v : rec[(7 + q) * n + nb];
+            const double dens = upIn[q] ? rhoIn : rhoEx;
+            const double total = -trans * dPhi * mob * dens;
+            for (int i = 0; i < NC; ++i)
+            {
+                const double wv = upIn[q] ? pr.w[q][i].v : rec[(10 + q * NC + i) * n + nb];
+                faceFlux[i] += total * wv;
+            }
+        }
+        for (int i = 0; i < NC; ++i) outward[i] += faceFlux[i];
+        if (!self)
+            continue;
+        // AD face flux with the recorded decision (face_flux.hpp:174-321)
+        S flux[NC];
+        for (int i = 0; i < NC; ++i) flux[i] = S(0.0);
+        for (int q = 0; q < 3; ++q)
+        {
+            const double rhoEx = rec[(1 + q) * n + nb];
+            const double satEx = rec[(4 + q) * n + nb];
+            const S satAvg = 0.5 * (pr.sat[q] + satEx);
+            const S avg = 0.5 * (pr.rho[q] * pr.sat[q] + rhoEx * satEx) / maxd(satAvg.v, 1.0e-8);
+            const S dPhi = pEx + avg * gdz - st.p;
+            if (upIn[q])
+            {
+                const S darcy = -trans * dPhi * pr.mob[q];
+                const S total = darcy * pr.rho[q];
+                for (int i = 0; i < NC; ++i) flux[i] += total * pr.w[q][i];
+            }
+            else
+            {
+                const double mobEx = rec[(7 + q) * n + nb];
+                const S darcy = -trans * dPhi * mobEx;
+                const S total = darcy * rhoEx;
+                for (int i = 0; i < NC; ++i)
+                    flux[i] += total * rec[(10 + q * NC + i) * n + nb];
+            }
+        }
+        double *rb = bsr + v.recipSlot[k] * NP * NP;
+        for (int i = 0; i < NC; ++i)
+            for (int q = 0; q < NP; ++q)
+            {
+                self[i * NP + q] += flux[i].d[q];
+                rb[i * NP + q] = -flux[i].d[q];
+            }
+        for (int r = NC; r < NP; ++r)
+            for (int q = 0; q < NP; ++q) rb[r * NP + q] = 0.0;
+    }
+    if (residual)
+    {
+        for (int i = 0; i < NC; ++i)
+            residual[c * NP + i] = res[i].v + outward[i];
+        for (int r = NC; r < NP; ++r)
+            residual[c * NP + r] = res[r].v;
+    }
+    if (status)
+        status[c] = NN_STATUS_OK;
+}
+// ------------------------------------------------------------ launchers
+template <int NC>
+hipError_t Launch<NC>::eos(unsigned grid, hipStream_t st, const FluidParams *fp, int64_t n,
+                           const double *P, const double *T, const double *x,
+                           const int32_t *role, double *Z, double *lnphi, double *fug,
+                           int32_t *status)
+{
+    eosPhaseKernel<NC><<<grid, kBlock, 0, st>>>(fp, n, P, T, x, role, Z, lnphi, fug, status);
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t Launch<NC>::flash(unsigned grid, hipStream_t st, const FluidParams *fp,
+                             const FlashOptions *op, int64_t n, const double *P, const double *T,
+                             const double *z, const uint8_t *allowed, const double *seed,
+                             nn_flash_out out, unsigned int *counter)
+{
+    flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, out, counter);
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t Launch<NC>::stability(unsigned grid, hipStream_t st, const FluidParams *fp,
+                                 const FlashOptions *op, int64_t n, const double *P,
+                                 const double *T, const double *z, const uint8_t *active,
+                                 const double *x, uint8_t *stable, uint8_t *valid,
+                                 uint8_t *mask, double *trial, double *incipient,
+                                 int32_t *status)
+{
+    stabilityKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, active, x, stable, valid,
+                                                 mask, trial, incipient, status);
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t Launch<NC>::phaseUpdate(bool well, unsigned grid, hipStream_t st,
+                                   const FluidParams *fp, const FlashOptions *op, int64_t n,
+                                   const double *T, double *primary, double *phaseState,
+                                   int32_t *status, uint8_t *flags, unsigned int *counter)
+{
+    if (well)
+        phaseUpdateKernel<NC, true><<<grid, kBlock, 0, st>>>(fp, op, n, T, primary, phaseState,
+                                                             status, flags, counter);
+    else
+        phaseUpdateKernel<NC, false><<<grid, kBlock, 0, st>>>(fp, op, n, T, primary, phaseState,
+                                                              status, flags, counter);
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t Launch<NC>::record(bool well, hipStream_t st, const FluidParams *fp, AssemblyView v,
+                              const double *primary, const double *phaseState, double *accum,
+                              int32_t *status)
+{
+    const unsigned grid = static_cast<unsigned>((v.ncell + 255) / 256);
+    if (well)
+        cellRecordKernel<NC, true><<<grid, 256, 0, st>>>(fp, v, primary, phaseState, accum, status);
+    else
+        cellRecordKernel<NC, false><<<grid, 256, 0, st>>>(fp, v, primary, phaseState, accum, status);
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t Launch<NC>::assemble(bool well, hipStream_t st, const FluidParams *fp, AssemblyView v,
+                                double dt, double fugScale, const double *primary,
+                                const double *phaseState, const double *accumPrev,
+                                double *residual, double *bsr, uint8_t *decisions,
+                                int32_t *status)
+{
+    const unsigned grid = static_cast<unsigned>((v.ncell + 63) / 64);
+    if (well)
+        assembleKernel<NC, true><<<grid, 64, 0, st>>>(fp, v, dt, fugScale, primary, phaseState,
+                                                      accumPrev, residual, bsr, decisions, status);
+    else
+        assembleKernel<NC, false><<<grid, 64, 0, st>>>(fp, v, dt, fugScale, primary, phaseState,
+                                                       accumPrev, residual, bsr, decisions, status);
+    return hipGetLastError();
+}
+
+} // namespace nn

@@ -1,0 +1,63 @@
+// launch.hpp — host-side launch interface of the per-component-count kernels.
+//
+// Each component count is compiled in its own translation unit
+// (nn_nc3.hip, nn_nc4.hip, nn_nc6.hip) that instantiates Launch<NC>; the C ABI
+// in nn_api.hip dispatches on the run-time nc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nonenone_amd.h"
+#include "common.hpp"
+
+namespace nn
+{
+
+constexpr int kBlock = 64;  // one wavefront per block: divergent per-state control flow
+
+struct AssemblyView
+{
+    int64_t ncell;
+    const int64_t *connOffsets;
+    const int64_t *connNeighbor;
+    const double *connTrans;
+    const double *connGdz;
+    const double *volume;
+    const double *porosity;
+    const double *T;
+    const int64_t *selfSlot;   // BSR slot of block (c, c)
+    const int64_t *recipSlot;  // per connection: BSR slot of block (neighbour, c)
+    double *record;            // neighbour property records, field-major (SoA)
+};
+
+template <int NC>
+struct Launch
+{
+    static hipError_t eos(unsigned grid, hipStream_t st, const FluidParams *fp, int64_t n,
+                          const double *P, const double *T, const double *x, const int32_t *role,
+                          double *Z, double *lnphi, double *fug, int32_t *status);
+    static hipError_t flash(unsigned grid, hipStream_t st, const FluidParams *fp,
+                            const FlashOptions *op, int64_t n, const double *P, const double *T,
+                            const double *z, const uint8_t *allowed, const double *seed,
+                            nn_flash_out out, unsigned int *counter);
+    static hipError_t stability(unsigned grid, hipStream_t st, const FluidParams *fp,
+                                const FlashOptions *op, int64_t n, const double *P,
+                                const double *T, const double *z, const uint8_t *active,
+                                const double *x, uint8_t *stable, uint8_t *valid, uint8_t *mask,
+                                double *trial, double *incipient, int32_t *status);
+    static hipError_t phaseUpdate(bool well, unsigned grid, hipStream_t st, const FluidParams *fp,
+                                  const FlashOptions *op, int64_t n, const double *T,
+                                  double *primary, double *phaseState, int32_t *status,
+                                  uint8_t *flags, unsigned int *counter);
+    static hipError_t record(bool well, hipStream_t st, const FluidParams *fp, AssemblyView v,
+                             const double *primary, const double *phaseState, double *accum,
+                             int32_t *status);
+    static hipError_t assemble(bool well, hipStream_t st, const FluidParams *fp, AssemblyView v,
+                               double dt, double fugScale, const double *primary,
+                               const double *phaseState, const double *accumPrev,
+                               double *residual, double *bsr, uint8_t *decisions,
+                               int32_t *status);
+};
+
+} // namespace nn

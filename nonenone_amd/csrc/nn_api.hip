@@ -1,0 +1,568 @@
+// nn_api.hip — the C ABI declared in include/nonenone_amd.h (kernels: kernels.hpp).
+//
+// One HIP work-item owns one state (flash, stability, phase update) or one
+// cell (assembly).  Flash-like kernels are persistent: a grid sized to the
+// chip pulls state indices from a device counter, so work-items that finish
+// early take new states instead of idling while slow states iterate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/nonenone_amd.h"
+#include "common.hpp"
+#include "launch.hpp"
+
+using namespace nn;
+
+struct nn_context
+{
+    int device = 0;
+    FluidParams host{};
+    FlashOptions hostOpts{};
+    FluidParams *dFluid = nullptr;
+    FlashOptions *dOpts = nullptr;
+    unsigned int *dCounter = nullptr;  // work-queue head for persistent kernels
+    int numCUs = 256;
+};
+
+namespace
+{
+
+thread_local char g_error[512] = "";
+
+int fail(int code, const char *msg)
+{
+    std::snprintf(g_error, sizeof(g_error), "%s", msg);
+    return code;
+}
+
+int hipFail(hipError_t e, const char *what)
+{
+    std::snprintf(g_error, sizeof(g_error), "%s: %s", what, hipGetErrorString(e));
+    return NN_E_HIP;
+}
+
+#define NN_HIP(call)                                  \
+    do                                                \
+    {                                                 \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hipFail(e_, #call); \
+    } while (0)
+
+
+template <class F>
+int dispatchNc(int nc, F &&f)
+{
+    switch (nc)
+    {
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 6: return f(std::integral_constant<int, 6>{});
+    default: return fail(NN_E_UNSUPPORTED, "component count not compiled into this library (3, 4, 6)");
+    }
+}
+
+
+unsigned int persistentGrid(const nn_context *ctx, int64_t n)
+{
+    // 16 wavefront-blocks per CU keeps every SIMD busy while the queue drains.
+    const int64_t cap = int64_t(ctx->numCUs) * 16;
+    const int64_t need = (n + kBlock - 1) / kBlock;
+    return static_cast<unsigned int>(std::max<int64_t>(1, std::min(cap, need)));
+}
+
+FluidParams toParams(const nn_fluid_desc &d)
+{
+    FluidParams p{};
+    p.nc = d.nc;
+    p.model = d.thermo_model;
+    p.eosType = d.eos_type;
+    p.water = d.water_component;
+    p.omegaA = d.omega_a;
+    p.omegaB = d.omega_b;
+    p.m1 = d.m1;
+    p.m2 = d.m2;
+    for (int i = 0; i < NN_MAX_NC; ++i)
+    {
+        p.tc[i] = d.tc[i];
+        p.pc[i] = d.pc[i];
+        p.vc[i] = d.vc[i];
+        p.omega[i] = d.acentric[i];
+        p.mw[i] = d.molar_mass[i];
+        for (int j = 0; j < NN_MAX_NC; ++j)
+        {
+            p.kij[i][j] = d.kij[i * NN_MAX_NC + j];
+            p.cpaCrossEps[i][j] = d.cpa_cross_epsilon[i * NN_MAX_NC + j];
+            p.cpaCrossBeta[i][j] = d.cpa_cross_beta[i * NN_MAX_NC + j];
+        }
+        p.swAqKij[i] = d.sw_aqueous_kij[i];
+        p.swAqSet[i] = d.sw_aqueous_kij_set[i];
+        p.cpaA0[i] = d.cpa_a0[i];
+        p.cpaB[i] = d.cpa_b[i];
+        p.cpaC1[i] = d.cpa_c1[i];
+        p.cpaEps[i] = d.cpa_epsilon[i];
+        p.cpaBeta[i] = d.cpa_beta[i];
+        p.cpaDonors[i] = d.cpa_donors[i];
+        p.cpaAcceptors[i] = d.cpa_acceptors[i];
+        // LBC constants, compositional_properties.hpp:166-182 (host libm, as the reference).
+        if (i < d.nc)
+        {
+            const double mwSqrt = std::sqrt(1000.0 * d.molar_mass[i]);
+            const double tcR = d.tc[i] / kKelvinPerRankine;
+            const double pcPsi = d.pc[i] / kPsi;
+            p.lbcMwSqrt[i] = mwSqrt;
+            p.lbcEps[i] = 5.4402 * std::pow(tcR, 1.0 / 6.0) /
+                          (mwSqrt * std::pow(pcPsi, 2.0 / 3.0) * kCentipoise);
+        }
+    }
+    p.swSalinity = d.sw_salinity;
+    for (int k = 0; k < 7; ++k) p.relperm[k] = d.relperm[k];
+    p.cpaTerm = d.cpa_physical_term;
+    p.cpaMaxIter = d.cpa_max_iterations;
+    p.cpaTol = d.cpa_tolerance;
+    p.cpaDamping = d.cpa_damping;
+    return p;
+}
+
+FlashOptions toOptions(const nn_flash_options &o)
+{
+    FlashOptions r{};
+    r.maxIterations = o.maximum_iterations;
+    r.maxRrIterations = o.maximum_rachford_rice_iterations;
+    r.maxStabilityIterations = o.maximum_stability_iterations;
+    r.stabilitySsi = o.stability_ssi_before_newton;
+    r.maxStabilityNewton = o.maximum_stability_newton;
+    r.twoPhaseSsi = o.two_phase_ssi_before_newton;
+    r.maxTwoPhaseNewton = o.maximum_two_phase_newton;
+    r.threePhaseSsi = o.three_phase_ssi_before_newton;
+    r.maxThreePhaseNewton = o.maximum_three_phase_newton;
+    r.fugTol = o.fugacity_tolerance;
+    r.rrTol = o.rachford_rice_tolerance;
+    r.stabTol = o.stability_tolerance;
+    r.betaTol = o.phase_fraction_tolerance;
+    r.coincidentTol = o.coincident_composition_tolerance;
+    r.floor = o.composition_floor;
+    r.logKStep = o.log_k_step_limit;
+    r.fdStep = o.numerical_jacobian_step;
+    return r;
+}
+
+} // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+void nn_flash_options_default(nn_flash_options *o)
+{
+    if (!o)
+        return;
+    o->maximum_iterations = 120;
+    o->maximum_rachford_rice_iterations = 80;
+    o->maximum_stability_iterations = 80;
+    o->stability_ssi_before_newton = 8;
+    o->maximum_stability_newton = 12;
+    o->two_phase_ssi_before_newton = 8;
+    o->maximum_two_phase_newton = 12;
+    o->three_phase_ssi_before_newton = 10;
+    o->maximum_three_phase_newton = 12;
+    o->fugacity_tolerance = 1.0e-9;
+    o->rachford_rice_tolerance = 1.0e-12;
+    o->stability_tolerance = 1.0e-9;
+    o->phase_fraction_tolerance = 1.0e-10;
+    o->coincident_composition_tolerance = 1.0e-8;
+    o->composition_floor = 1.0e-30;
+    o->log_k_step_limit = 2.0;
+    o->numerical_jacobian_step = 1.0e-6;
+}
+
+const char *nn_last_error(void) { return g_error; }
+
+int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, nn_context **out)
+{
+    if (!fluid || !out)
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (fluid->nc < 2 || fluid->nc > NN_MAX_NC)
+        return fail(NN_E_INVALID_ARGUMENT, "nc out of range");
+    if (fluid->eos_type != 1 && fluid->eos_type != 5)
+        return fail(NN_E_INVALID_ARGUMENT, "eos_type must be 1 or 5");
+    if (!(fluid->omega_a > 0.0) || !(fluid->omega_b > 0.0) || fluid->m1 == fluid->m2)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid cubic EOS constants");
+    if (fluid->thermo_model == NN_MODEL_CPA)
+        return fail(NN_E_UNSUPPORTED, "CPA is not provided by this build");
+    if (fluid->thermo_model == NN_MODEL_SW &&
+        (fluid->water_component < 0 || fluid->water_component >= fluid->nc))
+        return fail(NN_E_INVALID_ARGUMENT, "SW water-component index is out of range");
+    for (int i = 0; i < fluid->nc; ++i)
+        for (int j = 0; j < fluid->nc; ++j)
+            if (fluid->kij[i * NN_MAX_NC + j] != fluid->kij[j * NN_MAX_NC + i])
+                return fail(NN_E_INVALID_ARGUMENT, "k_ij must be symmetric");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(NN_E_NO_DEVICE, "no HIP device");
+    nn_context *ctx = new (std::nothrow) nn_context;
+    if (!ctx)
+        return fail(NN_E_OUT_OF_MEMORY, "host allocation failed");
+    NN_HIP(hipGetDevice(&ctx->device));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->numCUs = prop.multiProcessorCount;
+    nn_flash_options defaults;
+    nn_flash_options_default(&defaults);
+    ctx->host = toParams(*fluid);
+    ctx->hostOpts = toOptions(opts ? *opts : defaults);
+    if (hipMalloc(&ctx->dFluid, sizeof(FluidParams)) != hipSuccess ||
+        hipMalloc(&ctx->dOpts, sizeof(FlashOptions)) != hipSuccess ||
+        hipMalloc(&ctx->dCounter, sizeof(unsigned int)) != hipSuccess)
+    {
+        nn_context_destroy(ctx);
+        return fail(NN_E_OUT_OF_MEMORY, "device allocation failed");
+    }
+    NN_HIP(hipMemcpy(ctx->dFluid, &ctx->host, sizeof(FluidParams), hipMemcpyHostToDevice));
+    NN_HIP(hipMemcpy(ctx->dOpts, &ctx->hostOpts, sizeof(FlashOptions), hipMemcpyHostToDevice));
+    *out = ctx;
+    return NN_OK;
+}
+
+int nn_context_destroy(nn_context *ctx)
+{
+    if (!ctx)
+        return NN_OK;
+    if (ctx->dFluid) (void)hipFree(ctx->dFluid);
+    if (ctx->dOpts) (void)hipFree(ctx->dOpts);
+    if (ctx->dCounter) (void)hipFree(ctx->dCounter);
+    delete ctx;
+    return NN_OK;
+}
+
+int nn_eos_phase_batch(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
+                       const double *d_x, const int32_t *d_role, double *d_Z, double *d_lnphi,
+                       double *d_fug, int32_t *d_status, void *stream)
+{
+    if (!ctx || n < 0)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid context or size");
+    if (n == 0)
+        return NN_OK;
+    if (!d_P || !d_T || !d_x || !d_role || !d_Z || !d_lnphi || !d_fug || !d_status)
+        return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return dispatchNc(ctx->host.nc, [&](auto ncTag) {
+        constexpr int NC = decltype(ncTag)::value;
+        const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+        NN_HIP((Launch<NC>::eos(grid, st, ctx->dFluid, n, d_P, d_T, d_x, d_role, d_Z, d_lnphi,
+                                d_fug, d_status)));
+        return NN_OK;
+    });
+}
+
+static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
+                       const double *d_z, const uint8_t *d_allowed, const double *d_seed,
+                       const nn_flash_out *out, void *stream)
+{
+    if (!ctx || n < 0 || !out)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    if (n == 0)
+        return NN_OK;
+    if (!d_P || !d_T || !d_z || !out->presence || !out->converged || !out->status ||
+        !out->iterations || !out->beta || !out->saturation || !out->x || !out->Z ||
+        !out->rho_m || !out->k_go || !out->k_wo)
+        return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
+    if (n > int64_t(0xFFFFFFF0u))
+        return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const nn_flash_out o = *out;
+    return dispatchNc(ctx->host.nc, [&](auto ncTag) {
+        constexpr int NC = decltype(ncTag)::value;
+        NN_HIP(hipMemsetAsync(ctx->dCounter, 0, sizeof(unsigned int), st));
+        NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), st, ctx->dFluid, ctx->dOpts, n, d_P, d_T,
+                                  d_z, d_allowed, d_seed, o, ctx->dCounter)));
+        return NN_OK;
+    });
+}
+
+int nn_flash_batch(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
+                   const double *d_z, const nn_flash_out *out, void *stream)
+{
+    return launchFlash(ctx, n, d_P, d_T, d_z, nullptr, nullptr, out, stream);
+}
+
+int nn_flash_restricted_batch(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
+                              const double *d_z, const uint8_t *d_allowed, const double *d_seed,
+                              const nn_flash_out *out, void *stream)
+{
+    if (!d_allowed)
+        return fail(NN_E_INVALID_ARGUMENT, "allowed masks required");
+    return launchFlash(ctx, n, d_P, d_T, d_z, d_allowed, d_seed, out, stream);
+}
+
+int nn_stability_batch(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
+                       const double *d_z, const uint8_t *d_active, const double *d_x,
+                       uint8_t *d_stable, uint8_t *d_valid, uint8_t *d_unstable_mask,
+                       double *d_trial_sum, double *d_incipient, int32_t *d_status, void *stream)
+{
+    if (!ctx || n < 0)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    if (n == 0)
+        return NN_OK;
+    if (!d_P || !d_T || !d_z || !d_active || !d_x || !d_stable || !d_valid || !d_unstable_mask ||
+        !d_trial_sum || !d_incipient || !d_status)
+        return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return dispatchNc(ctx->host.nc, [&](auto ncTag) {
+        constexpr int NC = decltype(ncTag)::value;
+        const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+        NN_HIP((Launch<NC>::stability(grid, st, ctx->dFluid, ctx->dOpts, n, d_P, d_T, d_z, d_active,
+                                      d_x, d_stable, d_valid, d_unstable_mask, d_trial_sum,
+                                      d_incipient, d_status)));
+        return NN_OK;
+    });
+}
+
+int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, const double *d_T,
+                          double *d_primary, double *d_phase_state, int32_t *d_status,
+                          uint8_t *d_flags, void *stream)
+{
+    if (!ctx || n < 0)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    if (n == 0)
+        return NN_OK;
+    if (!d_T || !d_primary || !d_phase_state || !d_status)
+        return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
+    if (ctx->host.water < 0)
+        return fail(NN_E_INVALID_ARGUMENT, "three-phase update needs a water component");
+    if (n > int64_t(0xFFFFFFF0u))
+        return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return dispatchNc(ctx->host.nc, [&](auto ncTag) {
+        constexpr int NC = decltype(ncTag)::value;
+        NN_HIP(hipMemsetAsync(ctx->dCounter, 0, sizeof(unsigned int), st));
+        NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, persistentGrid(ctx, n), st,
+                                        ctx->dFluid, ctx->dOpts, n, d_T, d_primary, d_phase_state,
+                                        d_status, d_flags, ctx->dCounter)));
+        return NN_OK;
+    });
+}
+
+// ------------------------------------------------------------------ assembly
+int nn_structured_topology(int32_t nx, int32_t ny, int32_t nz, const double *dx, const double *dy,
+                           const double *dz, const double *perm, int64_t *conn_offsets,
+                           int64_t *conn_neighbor, double *conn_trans, double *conn_gdz,
+                           int64_t *row_offsets, int64_t *col_blocks, int64_t *nconn,
+                           int64_t *nnzb)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !dx || !dy || !dz || !perm || !nconn || !nnzb)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid grid description");
+    const int64_t ncell = int64_t(nx) * ny * nz;
+    // Vertical centres accumulate dz (structured_cartesian_geometry.hpp:60-71).
+    std::vector<double> zc(nz);
+    double pos = 0.0;
+    for (int k = 0; k < nz; ++k)
+    {
+        zc[k] = pos + 0.5 * dz[k];
+        pos += dz[k];
+    }
+    auto halfTrans = [&](int i, int j, int k, int dir) {
+        const int axis = dir / 2;
+        const double w[3] = {dx[i], dy[j], dz[k]};
+        const double kperm = perm[((int64_t(k) * ny + j) * nx + i) * 3 + axis];
+        const double dist = 0.5 * w[axis];
+        double normal = 0.0;
+        switch (dir)
+        {
+        case 0: normal = -w[1] * w[2]; break;
+        case 1: normal = w[1] * w[2]; break;
+        case 2: normal = -w[0] * w[2]; break;
+        case 3: normal = w[0] * w[2]; break;
+        case 4: normal = -w[0] * w[1]; break;
+        default: normal = w[0] * w[1]; break;
+        }
+        return kperm * std::fabs(normal) / dist;
+    };
+    int64_t nc_ = 0, nb_ = 0;
+    const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
+    std::vector<int64_t> cols;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+            {
+                const int64_t cell = (int64_t(k) * ny + j) * nx + i;
+                if (conn_offsets) conn_offsets[cell] = nc_;
+                cols.clear();
+                cols.push_back(cell);
+                for (int d = 0; d < 6; ++d)
+                {
+                    const int ni = i + di[d], nj = j + dj[d], nk = k + dk[d];
+                    if (ni < 0 || nj < 0 || nk < 0 || ni >= nx || nj >= ny || nk >= nz)
+                        continue;
+                    const int64_t nbr = (int64_t(nk) * ny + nj) * nx + ni;
+                    // grid/include/structuredgrid/structuredgrid.hpp:642-667
+                    const double h1 = halfTrans(i, j, k, d);
+                    const double h2 = halfTrans(ni, nj, nk, d ^ 1);
+                    const double t = (h1 <= 1.0e-30 || h2 <= 1.0e-30) ? 0.0 : 1.0 / (1.0 / h1 + 1.0 / h2);
+                    // structuredgrid.hpp:611-624: g * (z_i - z_j) on Z faces only
+                    const double g = (d >= 4) ? (zc[k] - zc[nk]) * 9.80665 : 0.0;
+                    if (conn_neighbor)
+                    {
+                        conn_neighbor[nc_] = nbr;
+                        conn_trans[nc_] = t;
+                        conn_gdz[nc_] = g;
+                    }
+                    ++nc_;
+                    cols.push_back(nbr);
+                }
+                std::sort(cols.begin(), cols.end());
+                if (row_offsets) row_offsets[cell] = nb_;
+                if (col_blocks)
+                    for (size_t q = 0; q < cols.size(); ++q) col_blocks[nb_ + q] = cols[q];
+                nb_ += int64_t(cols.size());
+            }
+    if (conn_offsets) conn_offsets[ncell] = nc_;
+    if (row_offsets) row_offsets[ncell] = nb_;
+    *nconn = nc_;
+    *nnzb = nb_;
+    return NN_OK;
+}
+
+} // extern "C"
+
+struct nn_assembly
+{
+    nn_context *ctx = nullptr;
+    int hasWell = 0;
+    AssemblyView view{};
+    int64_t nnzb = 0;
+    int64_t *dSelf = nullptr;
+    int64_t *dRecip = nullptr;
+    double *dRecord = nullptr;
+};
+
+extern "C" {
+
+int nn_assembly_destroy(nn_assembly *a)
+{
+    if (!a)
+        return NN_OK;
+    if (a->dSelf) (void)hipFree(a->dSelf);
+    if (a->dRecip) (void)hipFree(a->dRecip);
+    if (a->dRecord) (void)hipFree(a->dRecord);
+    delete a;
+    return NN_OK;
+}
+
+int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
+                       const int64_t *d_conn_offsets, const int64_t *d_conn_neighbor,
+                       const double *d_conn_trans, const double *d_conn_gdz,
+                       const int64_t *d_row_offsets, const int64_t *d_col_blocks, int64_t nnzb,
+                       const double *d_volume, const double *d_porosity, const double *d_T,
+                       nn_assembly **out)
+{
+    if (!ctx || !out || ncell <= 0 || !d_conn_offsets || !d_conn_neighbor || !d_conn_trans ||
+        !d_conn_gdz || !d_row_offsets || !d_col_blocks || !d_volume || !d_porosity || !d_T)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid assembly description");
+    *out = nullptr;
+    if (ctx->host.water < 0)
+        return fail(NN_E_INVALID_ARGUMENT, "Natural assembly needs a water component");
+    // Slots of the diagonal and reciprocal blocks in the cached pattern (setup only).
+    std::vector<int64_t> co(ncell + 1), ro(ncell + 1);
+    NN_HIP(hipMemcpy(co.data(), d_conn_offsets, sizeof(int64_t) * (ncell + 1), hipMemcpyDeviceToHost));
+    NN_HIP(hipMemcpy(ro.data(), d_row_offsets, sizeof(int64_t) * (ncell + 1), hipMemcpyDeviceToHost));
+    const int64_t nconn = co[ncell];
+    if (ro[ncell] != nnzb || nconn < 0)
+        return fail(NN_E_INVALID_ARGUMENT, "pattern size mismatch");
+    std::vector<int64_t> nbr(nconn), cols(nnzb);
+    NN_HIP(hipMemcpy(nbr.data(), d_conn_neighbor, sizeof(int64_t) * nconn, hipMemcpyDeviceToHost));
+    NN_HIP(hipMemcpy(cols.data(), d_col_blocks, sizeof(int64_t) * nnzb, hipMemcpyDeviceToHost));
+    auto slot = [&](int64_t row, int64_t col) -> int64_t {
+        if (row < 0 || row >= ncell) return -1;
+        const int64_t *b = cols.data() + ro[row], *e = cols.data() + ro[row + 1];
+        const int64_t *it = std::lower_bound(b, e, col);
+        return (it == e || *it != col) ? -1 : int64_t(it - cols.data());
+    };
+    std::vector<int64_t> self(ncell), recip(nconn);
+    std::vector<uint8_t> covered(nnzb, 0);
+    for (int64_t c = 0; c < ncell; ++c)
+    {
+        self[c] = slot(c, c);
+        if (self[c] < 0)
+            return fail(NN_E_INVALID_ARGUMENT, "pattern lacks a diagonal block");
+        covered[self[c]]++;
+        for (int64_t k = co[c]; k < co[c + 1]; ++k)
+        {
+            recip[k] = slot(nbr[k], c);
+            if (recip[k] < 0)
+                return fail(NN_E_INVALID_ARGUMENT, "pattern lacks a reciprocal block");
+            covered[recip[k]]++;
+        }
+    }
+    for (int64_t b = 0; b < nnzb; ++b)
+        if (covered[b] != 1)
+            return fail(NN_E_INVALID_ARGUMENT,
+                        "every pattern block must be a diagonal or one reciprocal face block");
+    nn_assembly *a = new (std::nothrow) nn_assembly;
+    if (!a)
+        return fail(NN_E_OUT_OF_MEMORY, "host allocation failed");
+    a->ctx = ctx;
+    a->hasWell = has_well_unknown ? 1 : 0;
+    a->nnzb = nnzb;
+    const int nc = ctx->host.nc;
+    const size_t recordBytes = sizeof(double) * size_t(10 + 3 * nc) * size_t(ncell);
+    if (hipMalloc(&a->dSelf, sizeof(int64_t) * ncell) != hipSuccess ||
+        hipMalloc(&a->dRecip, sizeof(int64_t) * std::max<int64_t>(1, nconn)) != hipSuccess ||
+        hipMalloc(&a->dRecord, recordBytes) != hipSuccess)
+    {
+        nn_assembly_destroy(a);
+        return fail(NN_E_OUT_OF_MEMORY, "device allocation failed");
+    }
+    NN_HIP(hipMemcpy(a->dSelf, self.data(), sizeof(int64_t) * ncell, hipMemcpyHostToDevice));
+    if (nconn > 0)
+        NN_HIP(hipMemcpy(a->dRecip, recip.data(), sizeof(int64_t) * nconn, hipMemcpyHostToDevice));
+    a->view = AssemblyView{ncell, d_conn_offsets, d_conn_neighbor, d_conn_trans, d_conn_gdz,
+                           d_volume, d_porosity, d_T, a->dSelf, a->dRecip, a->dRecord};
+    *out = a;
+    return NN_OK;
+}
+
+int nn_assembly_accumulation(nn_assembly *a, const double *d_primary, const double *d_phase_state,
+                             double *d_accum, void *stream)
+{
+    if (!a || !d_primary || !d_phase_state || !d_accum)
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    AssemblyView v = a->view;
+    v.record = nullptr;
+    return dispatchNc(a->ctx->host.nc, [&](auto ncTag) {
+        constexpr int NC = decltype(ncTag)::value;
+        NN_HIP((Launch<NC>::record(a->hasWell != 0, st, a->ctx->dFluid, v, d_primary, d_phase_state,
+                                   d_accum, nullptr)));
+        return NN_OK;
+    });
+}
+
+int nn_assembly_evaluate(nn_assembly *a, double dt, double fugacity_scaling,
+                         const double *d_primary, const double *d_phase_state,
+                         const double *d_accum_prev, double *d_residual, double *d_bsr,
+                         uint8_t *d_decisions, int32_t *d_status, void *stream)
+{
+    if (!a || !d_primary || !d_phase_state || !d_accum_prev)
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    if (!(dt > 0.0) || !(fugacity_scaling > 0.0))
+        return fail(NN_E_INVALID_ARGUMENT, "time step and fugacity scaling must be positive");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const AssemblyView v = a->view;
+    return dispatchNc(a->ctx->host.nc, [&](auto ncTag) {
+        constexpr int NC = decltype(ncTag)::value;
+        NN_HIP((Launch<NC>::record(a->hasWell != 0, st, a->ctx->dFluid, v, d_primary, d_phase_state,
+                                   nullptr, nullptr)));
+        NN_HIP((Launch<NC>::assemble(a->hasWell != 0, st, a->ctx->dFluid, v, dt, fugacity_scaling,
+                                     d_primary, d_phase_state, d_accum_prev, d_residual, d_bsr,
+                                     d_decisions, d_status)));
+        return NN_OK;
+    });
+}
+
+} // extern "C"

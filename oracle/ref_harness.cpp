@@ -1,0 +1,715 @@
+// ORACLE — test infrastructure only.  Never linked into, imported by or
+// called from the product library; only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load the shared object built from this file.
+//
+// ref_harness.cpp: a C-ABI wrapper around the reference implementation
+// itself.  It contains no reference code; it #includes the reference's
+// header-only C++17 sources where they lie (include paths set by
+// oracle/Makefile) and restates only the PETSc-bound driver loops that cannot
+// be compiled without PETSc:
+//   * the residual / Jacobian per-cell loops of
+//     models/include/natural/petsc/detail/natural_petsc_runtime_residual.inc:44-133
+//     models/include/natural/petsc/detail/natural_petsc_runtime_jacobian.inc:8-299
+//   * the cell-cache build of natural_petsc_runtime_cell_cache.inc:43-126
+//   * the previous-accumulation refresh of natural_petsc_runtime_cache_history.inc:177-260
+// Everything per cell (EOS, flash, stability, phase update, AD properties,
+// local residual, face flux) is the reference's own code.
+//
+// Output goes to oracle/_ref/libnn_ref.so (never committed).
+
+#include <indices/indices.hpp>
+#include <natural/assembly/cell_residual.hpp>
+#include <natural/fluid_system.hpp>
+#include <natural/kernel/cell_kernel.hpp>
+#include <natural/physics/accumulation.hpp>
+#include <natural/physics/face_flux.hpp>
+#include <natural/petsc/phase_state_codec.hpp>
+#include <natural/petsc/property_conversion.hpp>
+#include <natural/state/state_codec.hpp>
+#include <natural/state/three_phase_equilibrium.hpp>
+#include <natural/thermo/three_phase_flash.hpp>
+
+#include "../include/nonenone_amd.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+namespace
+{
+
+template <int NC, bool Well>
+using Cfg = MPMC::CompositionalModelConfig<
+    NC, true, Well, false, false, false,
+    MPMC::PhaseBehaviorModel::FullyCompositionalThreePhase>;
+template <int NC, bool Well>
+using AdIdx = MPMC::ADIndices<Cfg<NC, Well>>;
+template <int NC>
+using ScIdx = MPMC::ScalarIndices<Cfg<NC, false>>;
+
+template <class Indices>
+MPMC::FluidSystem<Indices> makeFluid(const nn_fluid_desc &d, double temperature)
+{
+    constexpr int N = Indices::numComponents;
+    using Arr = std::array<double, N>;
+    Arr tc{}, pc{}, vc{}, om{}, mw{};
+    std::array<Arr, N> kij{};
+    for (int i = 0; i < N; ++i)
+    {
+        tc[i] = d.tc[i];
+        pc[i] = d.pc[i];
+        vc[i] = d.vc[i];
+        om[i] = d.acentric[i];
+        mw[i] = d.molar_mass[i];
+        for (int j = 0; j < N; ++j)
+            kij[i][j] = d.kij[i * NN_MAX_NC + j];
+    }
+    MPMC::CompositionalMixture<Indices> mixture(tc, pc, vc, om, mw, kij);
+    MPMC::CubicEquationOfState<Indices> eos(
+        d.omega_a, d.omega_b, std::move(mixture), d.eos_type, d.m1, d.m2);
+    if (d.thermo_model == NN_MODEL_SW)
+    {
+        typename MPMC::CubicEquationOfState<Indices>::SoreideWhitsonOptions sw;
+        sw.waterComponent = d.water_component;
+        sw.salinityMolality = d.sw_salinity;
+        for (int c = 0; c < N; ++c)
+        {
+            if (d.sw_aqueous_kij_set[c])
+            {
+                const double k = d.sw_aqueous_kij[c];
+                sw.aqueousWaterBip[c] = [k](double, double) { return k; };
+            }
+        }
+        eos.configureSoreideWhitson(std::move(sw));
+    }
+    else if (d.thermo_model == NN_MODEL_CPA)
+    {
+        typename MPMC::CubicEquationOfState<Indices>::CubicPlusAssociationOptions cpa;
+        for (int c = 0; c < N; ++c)
+        {
+            cpa.a0[c] = d.cpa_a0[c];
+            cpa.b[c] = d.cpa_b[c];
+            cpa.c1[c] = d.cpa_c1[c];
+            cpa.associationEnergy[c] = d.cpa_epsilon[c];
+            cpa.associationVolume[c] = d.cpa_beta[c];
+            cpa.donorSites[c] = d.cpa_donors[c];
+            cpa.acceptorSites[c] = d.cpa_acceptors[c];
+            for (int e = 0; e < N; ++e)
+            {
+                cpa.crossAssociationEnergy[c][e] = d.cpa_cross_epsilon[c * NN_MAX_NC + e];
+                cpa.crossAssociationVolume[c][e] = d.cpa_cross_beta[c * NN_MAX_NC + e];
+            }
+        }
+        cpa.physicalTerm = d.cpa_physical_term == 1
+            ? MPMC::CpaCubicPhysicalTerm::PengRobinson
+            : MPMC::CpaCubicPhysicalTerm::SoaveRedlichKwong;
+        cpa.radialDistribution = MPMC::CpaRadialDistribution::Simplified;
+        if (d.cpa_max_iterations > 0)
+            cpa.maximumAssociationIterations = d.cpa_max_iterations;
+        if (d.cpa_tolerance > 0.0)
+            cpa.associationTolerance = d.cpa_tolerance;
+        if (d.cpa_damping > 0.0)
+            cpa.associationDamping = d.cpa_damping;
+        eos.configureCubicPlusAssociation(std::move(cpa));
+    }
+
+    typename MPMC::FluidSystem<Indices>::ComponentNames names{};
+    for (int i = 0; i < N; ++i)
+        names[i] = "c" + std::to_string(i);
+    typename MPMC::FluidSystem<Indices>::PhaseNames phases{"Oil", "Gas", "Water"};
+    MPMC::FluidSystem<Indices> fluid(
+        {730.0, 1.8, 985.4}, {2.4e-4, 1.86e-5, 4.69e-4}, std::move(eos),
+        names, phases, temperature);
+    if (d.water_component >= 0)
+        fluid.configureFullyCompositionalThreePhase(d.water_component);
+
+    using Eval = typename Indices::ValueType;
+    const double swc = d.relperm[0], sorw = d.relperm[1], sgc = d.relperm[2],
+                 sorg = d.relperm[3], krwMax = d.relperm[4], krgMax = d.relperm[5],
+                 kroMax = d.relperm[6];
+    // Corey forms of case/h2o_co2_nc10_2d_benchmark/benchmark_common.hpp:97-126.
+    fluid.waterRelativePermeability = [=](Eval sw) {
+        const double value = MPMC::scalarValue(sw);
+        if (value <= swc) return Eval(0.0);
+        if (value >= 1.0 - sorw) return Eval(krwMax);
+        const Eval se = (sw - swc) / (1.0 - swc - sorw);
+        return Eval(krwMax) * se * se;
+    };
+    fluid.gasRelativePermeability = [=](Eval sg) {
+        const double value = MPMC::scalarValue(sg);
+        if (value <= sgc) return Eval(0.0);
+        if (value >= 1.0 - sorg - swc) return Eval(krgMax);
+        const Eval se = (sg - sgc) / (1.0 - sorg - swc - sgc);
+        return Eval(krgMax) * se * se;
+    };
+    fluid.threePhaseOilRelativePermeability = [=](Eval, Eval so, Eval) {
+        const double value = MPMC::scalarValue(so);
+        if (value <= sorg) return Eval(0.0);
+        if (value >= 1.0 - swc) return Eval(kroMax);
+        const Eval se = (so - sorg) / (1.0 - swc - sorg);
+        return Eval(kroMax) * se * se;
+    };
+    return fluid;
+}
+
+MPMC::ThreePhaseFlashOptions makeOptions(const nn_flash_options &o, int water)
+{
+    MPMC::ThreePhaseFlashOptions r;
+    r.waterComponent = water;
+    r.maximumIterations = o.maximum_iterations;
+    r.maximumRachfordRiceIterations = o.maximum_rachford_rice_iterations;
+    r.maximumStabilityIterations = o.maximum_stability_iterations;
+    r.stabilitySsiIterationsBeforeNewton = o.stability_ssi_before_newton;
+    r.maximumStabilityNewtonIterations = o.maximum_stability_newton;
+    r.twoPhaseSsiIterationsBeforeNewton = o.two_phase_ssi_before_newton;
+    r.maximumTwoPhaseNewtonIterations = o.maximum_two_phase_newton;
+    r.threePhaseSsiIterationsBeforeNewton = o.three_phase_ssi_before_newton;
+    r.maximumThreePhaseNewtonIterations = o.maximum_three_phase_newton;
+    r.fugacityTolerance = o.fugacity_tolerance;
+    r.rachfordRiceTolerance = o.rachford_rice_tolerance;
+    r.stabilityTolerance = o.stability_tolerance;
+    r.phaseFractionTolerance = o.phase_fraction_tolerance;
+    r.coincidentPhaseCompositionTolerance = o.coincident_composition_tolerance;
+    r.compositionFloor = o.composition_floor;
+    r.logKStepLimit = o.log_k_step_limit;
+    r.numericalJacobianStep = o.numerical_jacobian_step;
+    return r;
+}
+
+struct Handle
+{
+    nn_fluid_desc desc;
+    nn_flash_options opts;
+};
+
+template <class F>
+void parallelFor(int64_t n, int threads, F &&body)
+{
+    if (threads <= 1 || n < 2)
+    {
+        body(0, int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int64_t chunk = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t)
+    {
+        const int64_t b = t * chunk;
+        const int64_t e = std::min(n, b + chunk);
+        if (b >= e)
+            break;
+        pool.emplace_back([&, t, b, e] { body(t, b, e); });
+    }
+    for (auto &th : pool)
+        th.join();
+}
+
+// ---------------------------------------------------------------- EOS
+template <int NC>
+int eosPhase(const Handle &h, int64_t n, const double *P, const double *T,
+             const double *x, const int32_t *role, double *Z, double *lnphi,
+             double *fug, int32_t *status)
+{
+    using I = ScIdx<NC>;
+    auto fluid = makeFluid<I>(h.desc, 300.0);
+    for (int64_t s = 0; s < n; ++s)
+    {
+        std::array<double, NC> comp{};
+        for (int c = 0; c < NC; ++c)
+            comp[c] = x[s * NC + c];
+        try
+        {
+            const auto r = fluid.eos.phaseResult(
+                P[s], T[s], comp, static_cast<MPMC::CompositionalPhase>(role[s]));
+            Z[s] = r.compressibility;
+            for (int c = 0; c < NC; ++c)
+            {
+                lnphi[s * NC + c] = std::log(r.fugacityCoefficient[c]);
+                fug[s * NC + c] = r.fugacity[c];
+            }
+            status[s] = NN_STATUS_OK;
+        }
+        catch (const std::exception &)
+        {
+            status[s] = NN_STATUS_THERMO_ERROR;
+        }
+    }
+    return NN_OK;
+}
+
+template <class Result, int NC>
+void storeFlash(const Result &r, int64_t s, uint8_t *presence, uint8_t *converged,
+                int32_t *iterations, double *beta, double *sat, double *x,
+                double *Z, double *rho, double *kgo, double *kwo)
+{
+    presence[s] = r.presence.bits();
+    converged[s] = r.converged ? 1 : 0;
+    iterations[s] = r.iterations;
+    for (int p = 0; p < 3; ++p)
+    {
+        beta[s * 3 + p] = r.phaseMoleFraction[p];
+        sat[s * 3 + p] = r.saturation[p];
+        Z[s * 3 + p] = r.compressibility[p];
+        rho[s * 3 + p] = r.molarDensity[p];
+        for (int c = 0; c < NC; ++c)
+            x[(s * 3 + p) * NC + c] = r.composition[p][c];
+    }
+    for (int c = 0; c < NC; ++c)
+    {
+        kgo[s * NC + c] = r.vaporOilK[c];
+        kwo[s * NC + c] = r.waterOilK[c];
+    }
+}
+
+template <int NC>
+int flashBatch(const Handle &h, int64_t n, const double *P, const double *T,
+               const double *z, const uint8_t *allowed, const double *seed,
+               uint8_t *presence, uint8_t *converged, int32_t *status,
+               int32_t *iterations, double *beta, double *sat, double *x,
+               double *Z, double *rho, double *kgo, double *kwo, int threads)
+{
+    using I = ScIdx<NC>;
+    parallelFor(n, threads, [&](int, int64_t b, int64_t e) {
+        auto fluid = makeFluid<I>(h.desc, 300.0);
+        MPMC::CubicThreePhaseFlash<I> flash(
+            fluid.eos, makeOptions(h.opts, h.desc.water_component));
+        for (int64_t s = b; s < e; ++s)
+        {
+            std::array<double, NC> comp{};
+            for (int c = 0; c < NC; ++c)
+                comp[c] = z[s * NC + c];
+            try
+            {
+                typename MPMC::CubicThreePhaseFlash<I>::Result r;
+                if (allowed == nullptr)
+                    r = flash.flash(P[s], T[s], comp);
+                else if (seed == nullptr)
+                    r = flash.flashRestricted(P[s], T[s], comp,
+                                              MPMC::PhasePresence(allowed[s]));
+                else
+                {
+                    std::array<std::array<double, NC>, 3> sd{};
+                    for (int p = 0; p < 3; ++p)
+                        for (int c = 0; c < NC; ++c)
+                            sd[p][c] = seed[(s * 3 + p) * NC + c];
+                    r = flash.flashRestricted(P[s], T[s], comp,
+                                              MPMC::PhasePresence(allowed[s]), sd);
+                }
+                storeFlash<decltype(r), NC>(r, s, presence, converged, iterations,
+                                            beta, sat, x, Z, rho, kgo, kwo);
+                status[s] = r.converged ? NN_STATUS_OK : NN_STATUS_NOT_CONVERGED;
+            }
+            catch (const std::invalid_argument &)
+            {
+                status[s] = NN_STATUS_INVALID_INPUT;
+                converged[s] = 0;
+            }
+            catch (const std::exception &)
+            {
+                status[s] = NN_STATUS_THERMO_ERROR;
+                converged[s] = 0;
+            }
+        }
+    });
+    return NN_OK;
+}
+
+template <int NC>
+int stabilityBatch(const Handle &h, int64_t n, const double *P, const double *T,
+                   const double *z, const uint8_t *active, const double *xin,
+                   uint8_t *stable, uint8_t *valid, uint8_t *mask, double *trial,
+                   double *incipient, int32_t *status)
+{
+    using I = ScIdx<NC>;
+    auto fluid = makeFluid<I>(h.desc, 300.0);
+    MPMC::CubicThreePhaseFlash<I> flash(
+        fluid.eos, makeOptions(h.opts, h.desc.water_component));
+    for (int64_t s = 0; s < n; ++s)
+    {
+        std::array<double, NC> comp{};
+        std::array<std::array<double, NC>, 3> xs{};
+        for (int c = 0; c < NC; ++c)
+            comp[c] = z[s * NC + c];
+        for (int p = 0; p < 3; ++p)
+            for (int c = 0; c < NC; ++c)
+                xs[p][c] = xin[(s * 3 + p) * NC + c];
+        try
+        {
+            const auto r = flash.stabilityTest(
+                P[s], T[s], comp, MPMC::PhasePresence(active[s]), xs);
+            stable[s] = r.stable;
+            valid[s] = r.valid;
+            uint8_t m = 0;
+            for (int p = 0; p < 3; ++p)
+            {
+                if (r.missingPhaseUnstable[p])
+                    m |= uint8_t(1u << p);
+                trial[s * 3 + p] = r.trialSum[p];
+                for (int c = 0; c < NC; ++c)
+                    incipient[(s * 3 + p) * NC + c] = r.incipientComposition[p][c];
+            }
+            mask[s] = m;
+            status[s] = NN_STATUS_OK;
+        }
+        catch (const std::invalid_argument &)
+        {
+            status[s] = NN_STATUS_INVALID_INPUT;
+        }
+        catch (const std::exception &)
+        {
+            status[s] = NN_STATUS_THERMO_ERROR;
+        }
+    }
+    return NN_OK;
+}
+
+// ---------------------------------------------------------- phase update
+template <int NC, bool Well>
+int phaseUpdate(const Handle &h, int64_t n, const double *T, double *primary,
+                double *phaseState, int32_t *status, uint8_t *flags, int threads)
+{
+    using I = AdIdx<NC, Well>;
+    constexpr int NP = I::numPrimaryVariables;
+    constexpr int NPS = I::numPhaseStateVariables;
+    parallelFor(n, threads, [&](int, int64_t b, int64_t e) {
+        auto fluid = makeFluid<I>(h.desc, 300.0);
+        MPMC::FullyCompositionalThreePhaseEquilibrium<I> eq(fluid);
+        for (int64_t s = b; s < e; ++s)
+        {
+            fluid.temperature = T[s];
+            std::array<double, NP> prim{};
+            std::array<double, NPS> ps{};
+            for (int k = 0; k < NP; ++k)
+                prim[k] = primary[s * NP + k];
+            for (int k = 0; k < NPS; ++k)
+                ps[k] = phaseState[s * NPS + k];
+            try
+            {
+                auto data = MPMC::PetscPhaseStateCodec<I>::decode(ps);
+                // natural_petsc_runtime_state.inc:215-221 (useVariableBounds = false)
+                eq.sanitizePrimaryBeforeFlash(prim, false);
+                const auto r = eq.updatePhaseState(prim, data);
+                const auto enc = MPMC::PetscPhaseStateCodec<I>::encode(data);
+                for (int k = 0; k < NP; ++k)
+                    primary[s * NP + k] = prim[k];
+                for (int k = 0; k < NPS; ++k)
+                    phaseState[s * NPS + k] = enc[k];
+                status[s] = r.status == MPMC::PhaseUpdateStatus::RecoverableThermodynamicFailure
+                    ? NN_STATUS_RECOVERABLE_FAILURE
+                    : NN_STATUS_OK;
+                if (flags)
+                {
+                    uint8_t f = 0;
+                    if (r.phaseRemoved) f |= 1;
+                    if (r.stabilityInvalid) f |= 2;
+                    if (r.missingPhaseUnstable) f |= 4;
+                    if (r.restrictedFlashFailed) f |= 8;
+                    if (r.unrestrictedFlashFailed) f |= 16;
+                    if (r.status == MPMC::PhaseUpdateStatus::StableReducedSet) f |= 32;
+                    flags[s] = f;
+                }
+            }
+            catch (const std::invalid_argument &)
+            {
+                status[s] = NN_STATUS_INVALID_INPUT;
+            }
+            catch (const std::exception &)
+            {
+                status[s] = NN_STATUS_THERMO_ERROR;
+            }
+        }
+    });
+    return NN_OK;
+}
+
+// ------------------------------------------------------------ assembly
+struct Topology
+{
+    int64_t ncell;
+    const int64_t *connOffsets;
+    const int64_t *connNeighbor;
+    const double *connTrans;
+    const double *connGdz;
+    const int64_t *rowOffsets;
+    const int64_t *colBlocks;
+    const double *volume;
+    const double *porosity;
+    const double *T;
+};
+
+int64_t findSlot(const Topology &t, int64_t row, int64_t col)
+{
+    const int64_t *b = t.colBlocks + t.rowOffsets[row];
+    const int64_t *e = t.colBlocks + t.rowOffsets[row + 1];
+    const int64_t *it = std::lower_bound(b, e, col);
+    if (it == e || *it != col)
+        throw std::logic_error("BSR pattern lacks a required block");
+    return it - t.colBlocks;
+}
+
+template <int NC, bool Well>
+struct AssemblyRunner
+{
+    using I = AdIdx<NC, Well>;
+    using Kernel = MPMC::NaturalCellKernel<I>;
+    using State = typename Kernel::State;
+    using Props = typename Kernel::Properties;
+    using Scalar = typename I::ValueType;
+    static constexpr int NP = I::numPrimaryVariables;
+    static constexpr int NPS = I::numPhaseStateVariables;
+
+    const Handle &h;
+    const Topology &t;
+    MPMC::FluidSystem<I> fluid;
+    Kernel kernel;
+    std::vector<State> states;
+    std::vector<Props> props;
+
+    AssemblyRunner(const Handle &hh, const Topology &tt)
+        : h(hh), t(tt), fluid(makeFluid<I>(hh.desc, 300.0)), kernel(fluid)
+    {
+    }
+
+    // natural_petsc_runtime_cell_cache.inc:58-116 (every local block).
+    void buildCache(const double *primary, const double *phaseState)
+    {
+        states.resize(t.ncell);
+        props.resize(t.ncell);
+        for (int64_t c = 0; c < t.ncell; ++c)
+        {
+            fluid.temperature = t.T[c];
+            std::array<double, NP> prim{};
+            std::array<double, NPS> ps{};
+            for (int k = 0; k < NP; ++k)
+                prim[k] = primary[c * NP + k];
+            for (int k = 0; k < NPS; ++k)
+                ps[k] = phaseState[c * NPS + k];
+            const auto data = MPMC::PetscPhaseStateCodec<I>::decode(ps);
+            states[c] = kernel.decodeState(prim, data);
+            std::array<Scalar, NC> overall{};
+            for (int k = 0; k < NC; ++k)
+                overall[k] = Scalar(data.overallComposition[k]);
+            props[c] = kernel.evaluateProperties(
+                states[c], Scalar(t.porosity[c]), 0.0, 2.0, &overall);
+        }
+    }
+
+    void accumulation(double *accum)
+    {
+        for (int64_t c = 0; c < t.ncell; ++c)
+        {
+            const auto scalar = MPMC::scalarizeCellProperties<I>(props[c]);
+            const auto a = MPMC::computeFluidAccumulation<I>(scalar);
+            for (int k = 0; k < NC; ++k)
+                accum[c * NC + k] = a.componentMass[k];
+        }
+    }
+
+    void evaluate(double dt, double fugScale, const double *accumPrev,
+                  double *residual, double *bsr, uint8_t *decisionsOut)
+    {
+        constexpr int N = NP;
+        std::vector<MPMC::FaceFluxLinearizationDecision<I>> decisions;
+        std::vector<double> selfBlock(N * N);
+        if (bsr)
+            std::fill(bsr, bsr + (t.rowOffsets[t.ncell]) * N * N, 0.0);
+        for (int64_t c = 0; c < t.ncell; ++c)
+        {
+            MPMC::CellProperties<I, double> prevProps{};
+            MPMC::AccumulationResult<I, double> prevAccum{};
+            for (int k = 0; k < NC; ++k)
+                prevAccum.componentMass[k] = accumPrev[c * NC + k];
+            const int64_t cb = t.connOffsets[c], ce = t.connOffsets[c + 1];
+            decisions.assign(static_cast<std::size_t>(ce - cb), {});
+
+            // residual.inc:59-120: scalar face fluxes record upwind decisions
+            MPMC::FaceMassFlux<I, double> outward{};
+            for (int64_t k = cb; k < ce; ++k)
+            {
+                const int64_t nb = t.connNeighbor[k];
+                const auto flux = MPMC::computeFaceMassFluxValue<I, Scalar>(
+                    states[c], props[c], states[nb], props[nb], t.connTrans[k],
+                    t.connGdz[k], t.volume[c], t.volume[nb], c, nb, -1,
+                    &decisions[static_cast<std::size_t>(k - cb)]);
+                for (int q = 0; q < NC; ++q)
+                    outward.component[q] += flux.component[q];
+                if (decisionsOut)
+                    for (int p = 0; p < 3; ++p)
+                        decisionsOut[k * 3 + p] = static_cast<uint8_t>(
+                            decisions[static_cast<std::size_t>(k - cb)].upwind[p]);
+            }
+            auto block = kernel.assembleLocalResidual(
+                states[c], props[c], prevProps, t.volume[c], dt, fugScale, -1,
+                2650.0, nullptr, &prevAccum, nullptr);
+            if (residual)
+            {
+                auto withFlux = block;
+                std::array<Scalar, NC> noWell{};
+                MPMC::addCellConservationTransportAndWell<I>(
+                    withFlux, outward, noWell, Scalar(0.0));
+                for (int r = 0; r < N; ++r)
+                    residual[c * N + r] = MPMC::scalarValue(withFlux.value[r]);
+            }
+            if (!bsr)
+                continue;
+            // jacobian.inc:240-271 self block (scaling disabled)
+            for (int r = 0; r < N; ++r)
+                for (int q = 0; q < N; ++q)
+                    selfBlock[r * N + q] = block.value[r].derivative(q);
+            // jacobian.inc:120-237 face AD pass, reusing residual decisions
+            for (int64_t k = cb; k < ce; ++k)
+            {
+                const int64_t nb = t.connNeighbor[k];
+                const auto flux = MPMC::computeFaceMassFlux<I, Scalar>(
+                    states[c], props[c], states[nb], props[nb], t.connTrans[k],
+                    t.connGdz[k], t.volume[c], t.volume[nb], c, nb, -1,
+                    &decisions[static_cast<std::size_t>(k - cb)]);
+                double *recip = bsr + findSlot(t, nb, c) * N * N;
+                for (int q = 0; q < NC; ++q)
+                {
+                    const int row = I::Equation::massConservation[q];
+                    for (int col = 0; col < N; ++col)
+                    {
+                        const double dv = flux.component[q].derivative(col);
+                        selfBlock[row * N + col] += dv;
+                        recip[row * N + col] = -dv;
+                    }
+                }
+            }
+            double *self = bsr + findSlot(t, c, c) * N * N;
+            for (int e = 0; e < N * N; ++e)
+                self[e] += selfBlock[e];
+        }
+    }
+};
+
+template <int NC, bool Well>
+int assemblyRun(const Handle &h, const Topology &t, double dt, double fugScale,
+                const double *primary, const double *phaseState,
+                const double *accumPrev, double *residual, double *bsr,
+                uint8_t *decisions, double *accumOut)
+{
+    AssemblyRunner<NC, Well> runner(h, t);
+    runner.buildCache(primary, phaseState);
+    if (accumOut)
+        runner.accumulation(accumOut);
+    if (residual || bsr)
+        runner.evaluate(dt, fugScale, accumPrev, residual, bsr, decisions);
+    return NN_OK;
+}
+
+template <class F>
+int dispatchNc(int nc, F &&f)
+{
+    switch (nc)
+    {
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 6: return f(std::integral_constant<int, 6>{});
+    default: return NN_E_UNSUPPORTED;
+    }
+}
+
+thread_local char g_error[512] = "";
+
+} // namespace
+
+extern "C" {
+
+void *ref_create(const nn_fluid_desc *desc, const nn_flash_options *opts)
+{
+    auto *h = new Handle;
+    h->desc = *desc;
+    h->opts = *opts;
+    return h;
+}
+
+void ref_destroy(void *h) { delete static_cast<Handle *>(h); }
+
+const char *ref_last_error(void) { return g_error; }
+
+int ref_eos_phase(void *hv, int64_t n, const double *P, const double *T,
+                  const double *x, const int32_t *role, double *Z, double *lnphi,
+                  double *fug, int32_t *status)
+{
+    const Handle &h = *static_cast<Handle *>(hv);
+    return dispatchNc(h.desc.nc, [&](auto nc) {
+        return eosPhase<decltype(nc)::value>(h, n, P, T, x, role, Z, lnphi, fug, status);
+    });
+}
+
+int ref_flash(void *hv, int64_t n, const double *P, const double *T, const double *z,
+              const uint8_t *allowed, const double *seed, uint8_t *presence,
+              uint8_t *converged, int32_t *status, int32_t *iterations,
+              double *beta, double *sat, double *x, double *Z, double *rho,
+              double *kgo, double *kwo, int threads)
+{
+    const Handle &h = *static_cast<Handle *>(hv);
+    return dispatchNc(h.desc.nc, [&](auto nc) {
+        return flashBatch<decltype(nc)::value>(
+            h, n, P, T, z, allowed, seed, presence, converged, status, iterations,
+            beta, sat, x, Z, rho, kgo, kwo, threads);
+    });
+}
+
+int ref_stability(void *hv, int64_t n, const double *P, const double *T,
+                  const double *z, const uint8_t *active, const double *x,
+                  uint8_t *stable, uint8_t *valid, uint8_t *mask, double *trial,
+                  double *incipient, int32_t *status)
+{
+    const Handle &h = *static_cast<Handle *>(hv);
+    return dispatchNc(h.desc.nc, [&](auto nc) {
+        return stabilityBatch<decltype(nc)::value>(
+            h, n, P, T, z, active, x, stable, valid, mask, trial, incipient, status);
+    });
+}
+
+int ref_phase_update(void *hv, int64_t n, int hasWell, const double *T,
+                     double *primary, double *phaseState, int32_t *status,
+                     uint8_t *flags, int threads)
+{
+    const Handle &h = *static_cast<Handle *>(hv);
+    return dispatchNc(h.desc.nc, [&](auto nc) {
+        constexpr int NC = decltype(nc)::value;
+        return hasWell
+            ? phaseUpdate<NC, true>(h, n, T, primary, phaseState, status, flags, threads)
+            : phaseUpdate<NC, false>(h, n, T, primary, phaseState, status, flags, threads);
+    });
+}
+
+int ref_assembly(void *hv, int64_t ncell, int hasWell, const int64_t *connOffsets,
+                 const int64_t *connNeighbor, const double *connTrans,
+                 const double *connGdz, const int64_t *rowOffsets,
+                 const int64_t *colBlocks, const double *volume,
+                 const double *porosity, const double *T, double dt,
+                 double fugScale, const double *primary, const double *phaseState,
+                 const double *accumPrev, double *residual, double *bsr,
+                 uint8_t *decisions, double *accumOut)
+{
+    const Handle &h = *static_cast<Handle *>(hv);
+    Topology t{ncell, connOffsets, connNeighbor, connTrans, connGdz, rowOffsets,
+               colBlocks, volume, porosity, T};
+    try
+    {
+        return dispatchNc(h.desc.nc, [&](auto nc) {
+            constexpr int NC = decltype(nc)::value;
+            return hasWell
+                ? assemblyRun<NC, true>(h, t, dt, fugScale, primary, phaseState,
+                                        accumPrev, residual, bsr, decisions, accumOut)
+                : assemblyRun<NC, false>(h, t, dt, fugScale, primary, phaseState,
+                                         accumPrev, residual, bsr, decisions, accumOut);
+        });
+    }
+    catch (const std::exception &e)
+    {
+        std::snprintf(g_error, sizeof(g_error), "%s", e.what());
+        return NN_E_INVALID_ARGUMENT;
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,183 @@
+"""Flash / stability / EOS parity: HIP kernels against the reference.
+
+Oracles, in order of strength:
+  * the reference itself, compiled from its sources (oracle/_ref, when built);
+  * golden fixtures in tests/golden/ written by tests/golden/make_golden.py
+    from that build (always present);
+  * the reference's own known answers (test/src/unit/natural_core_test.cpp:84-125)
+    and physical invariants (test/src/unit/three_phase_flash_test.cpp:116-191).
+Bar: phase labels / status bit-exact, values within 1e-10 relative.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from nonenone_amd import fluids
+from nonenone_amd._abi import FlashOptions, FluidDesc, NN_MAX_NC, NN_MODEL_PR
+from nonenone_amd.workloads import batched_flash_states
+
+import flashcmp
+import refharness
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+CASES = {
+    "flash_nc3_pr": lambda: fluids.h2o_co2_nc10("PR"),
+    "flash_nc3_sw": lambda: fluids.h2o_co2_nc10("SW"),
+    "flash_nc6_pr": lambda: fluids.six_component("PR"),
+    "flash_nc6_sw": lambda: fluids.six_component("SW"),
+}
+
+
+def core_test_fluid() -> FluidDesc:
+    """The 6-component PR mixture of natural_core_test.cpp:58-82 (eos type 1)."""
+    d = FluidDesc()
+    d.nc = 6
+    d.thermo_model = NN_MODEL_PR
+    d.eos_type = 1
+    d.water_component = -1
+    d.omega_a, d.omega_b = 0.4572355, 0.0779691
+    d.m1, d.m2 = 2.414213562373095, -0.414213562373095
+    tc = [189.515, 304.2, 387.607, 597.497, 698.515, 875.0]
+    pc = [4580011.59, 7386592.50, 4095515.97, 3345244.875, 1768374.5625, 1169006.79]
+    vc = [9.97012032965401e-5, 9.26344713533338e-5, 0.000217076707259486,
+          0.000381162235869935, 0.000721410148917871, 0.00113570073874421]
+    om = [0.00854, 0.228, 0.16733, 0.38609, 0.80784, 1.23141]
+    mw = [0.0161594, 0.04401, 0.0455725, 0.11774, 0.248827, 0.48152]
+    k = [[0.0, 0.00070981, 0.00077754, 0.0100, 0.0110, 0.0110],
+         [0.00070981, 0.0, 0.1500, 0.1500, 0.1500, 0.1500],
+         [0.00077754, 0.1500, 0.0, 0.0, 0.0, 0.0],
+         [0.0100, 0.1500, 0.0, 0.0, 0.0, 0.0],
+         [0.0110, 0.1500, 0.0, 0.0, 0.0, 0.0],
+         [0.0110, 0.1500, 0.0, 0.0, 0.0, 0.0]]
+    for i in range(6):
+        d.tc[i], d.pc[i], d.vc[i], d.acentric[i], d.molar_mass[i] = tc[i], pc[i], vc[i], om[i], mw[i]
+        for j in range(6):
+            d.kij[i * NN_MAX_NC + j] = k[i][j]
+    return d
+
+
+# natural_core_test.cpp:86-108
+CORE_X = [0.3246914, 0.0128351, 0.2278401, 0.2606985, 0.1134144, 0.0605206]
+CORE_Y = [0.808671, 0.025284, 0.1487798, 0.0175878, 0.0006759, 1.51e-06]
+CORE_ZL, CORE_ZV = 0.66880987807546899, 0.84105046763526459
+CORE_FL = [11148938.7289865, 285739.262764987, 958802.358789237, 32132.229549948,
+           222.962965571357, 0.0303098653818211]
+CORE_FV = [11364784.8024743, 289457.590456904, 964022.546010914, 31595.5016348791,
+           198.090892606485, 0.0143840907404893]
+
+
+def load_case(name):
+    data = np.load(GOLDEN / f"{name}.npz")
+    return {k: data[k] for k in data.files}
+
+
+# ------------------------------------------------------------------ CPU tests
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_fixture_invariants(name):
+    """The committed fixtures satisfy the flash invariants of three_phase_flash_test.cpp:116-191."""
+    g = load_case(name)
+    conv = g["converged"] == 1
+    assert conv.mean() > 0.95
+    beta = g["beta"][conv]
+    x = g["x"][conv]
+    z = g["z"][conv]
+    z = z / z.sum(axis=1, keepdims=True)
+    assert np.all(np.abs(beta.sum(axis=1) - 1.0) < 1e-10)
+    assert np.all(np.abs(g["saturation"][conv].sum(axis=1) - 1.0) < 1e-10)
+    active = beta > 1e-10
+    rec = np.einsum("np,npc->nc", np.where(active, beta, 0.0), x)
+    assert np.max(np.abs(rec - z)) < 2e-8
+    for p in range(3):
+        on = active[:, p]
+        assert np.all(np.abs(x[on, p].sum(axis=1) - 1.0) < 1e-10)
+    # presence bits follow the phase fractions
+    bits = (active * np.array([1, 2, 4])).sum(axis=1)
+    assert np.array_equal(bits, g["presence"][conv])
+
+
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_fixture_matches_reference_build(name):
+    g = load_case(name)
+    ref = refharness.Reference(CASES[name](), FlashOptions.defaults())
+    out = ref.flash(g["P"], g["T"], g["z"])
+    for k in ("presence", "converged", "status", "iterations", "beta", "x", "Z"):
+        assert np.array_equal(out[k], g[k]), k
+
+
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+def test_reference_build_reproduces_core_test_goldens():
+    ref = refharness.Reference(core_test_fluid(), FlashOptions.defaults())
+    P = np.full(2, 15.0e6)
+    T = np.full(2, 387.45)
+    Z, lnphi, fug, st = ref.eos_phase(P, T, np.array([CORE_X, CORE_Y]), np.array([0, 1]))
+    assert np.all(st == 0)
+    assert abs(Z[0] - CORE_ZL) < 2e-12 and abs(Z[1] - CORE_ZV) < 2e-12
+    np.testing.assert_allclose(fug[0], CORE_FL, rtol=2e-8)
+    np.testing.assert_allclose(fug[1], CORE_FV, rtol=2e-8)
+
+
+# ------------------------------------------------------------------ GPU tests
+def _flash_gpu(fluid):
+    from nonenone_amd.flash import ThreePhaseFlash
+    return ThreePhaseFlash(fluid)
+
+
+@pytest.mark.gpu
+def test_gpu_eos_core_test_goldens():
+    """E1-E5 against natural_core_test.cpp:96-108 (Z 2e-12, fugacities 2e-8)."""
+    fl = _flash_gpu(core_test_fluid())
+    P = np.full(2, 15.0e6)
+    T = np.full(2, 387.45)
+    Z, lnphi, fug, st = fl.eos_phase(P, T, np.array([CORE_X, CORE_Y]), np.array([0, 1]))
+    Z, fug, st = Z.cpu().numpy(), fug.cpu().numpy(), st.cpu().numpy()
+    assert np.all(st == 0)
+    assert abs(Z[0] - CORE_ZL) < 2e-12
+    assert abs(Z[1] - CORE_ZV) < 2e-12
+    np.testing.assert_allclose(fug[0], CORE_FL, rtol=2e-8)
+    np.testing.assert_allclose(fug[1], CORE_FV, rtol=2e-8)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_flash_matches_golden(name):
+    g = load_case(name)
+    fl = _flash_gpu(CASES[name]())
+    mine = flashcmp.to_numpy(fl.flash(g["P"], g["T"], g["z"]))
+    n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, g)
+    print(f"{name}: status mismatches {n_status}, label mismatches {n_label}, "
+          f"max rel diff {worst:.3e} (state {wi}), compared {int(sel.sum())}")
+    assert n_status == 0
+    assert n_label == 0
+    assert worst <= flashcmp.REL_TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_flash_matches_reference_build(name):
+    fluid = CASES[name]()
+    n = 1024 if "pr" in name else 256
+    P, T, z = batched_flash_states(n, fluid, seed=7, near_critical_fraction=0.1)
+    ref = refharness.Reference(fluid, FlashOptions.defaults()).flash(P, T, z, threads=8)
+    mine = flashcmp.to_numpy(_flash_gpu(fluid).flash(P, T, z))
+    n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, ref)
+    print(f"{name}: n={n} status mismatches {n_status}, label mismatches {n_label}, "
+          f"max rel diff {worst:.3e}")
+    assert n_status == 0 and n_label == 0
+    assert worst <= flashcmp.REL_TOL
+
+
+@pytest.mark.gpu
+def test_gpu_stability_matches_golden_single_phase():
+    """stabilityTest of each converged single-phase golden state reports stable."""
+    g = load_case("flash_nc3_pr")
+    fl = _flash_gpu(fluids.h2o_co2_nc10("PR"))
+    sel = (g["converged"] == 1) & np.isin(g["presence"], [1, 2, 4])
+    r = fl.stability_test(g["P"][sel], g["T"][sel], g["z"][sel], g["presence"][sel], g["x"][sel])
+    assert np.all(r["status"].cpu().numpy() == 0)
+    assert np.all(r["valid"].cpu().numpy() == 1)
+    assert np.all(r["stable"].cpu().numpy() == 1)
