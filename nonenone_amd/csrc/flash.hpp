@@ -27,6 +27,11 @@
 #include "common.hpp"
 #include "eos.hpp"
 
+// Optional control-flow trace (printf-style), compiled out unless defined.
+#ifndef NN_TRACE
+#define NN_TRACE(...) ((void)0)
+#endif
+
 namespace nn
 {
 
@@ -1551,6 +1556,7 @@ public:
     // canonicalization.inc:732-810
     __host__ __device__ void canonicalizeFinal(double P, double T, Result &r, bool preserve = false)
     {
+        NN_TRACE("final in conv=%d pres=%d beta=%.17g %.17g %.17g\n", (int)r.converged, (int)r.presence, r.beta[0], r.beta[1], r.beta[2]);
         if (!r.converged)
             return;
         if (sw())
@@ -1901,6 +1907,7 @@ public:
             const double xi = maxd(refComp[c], o.floor);
             const double phi = maxd(refThermo.phi[c], 1.0e-300);
             logRef[c] = ::log(xi) + ::log(phi);
+            NN_TRACE("stab ref=%d c=%d x=%.17g phi=%.17g logRef=%.17g\n", ref, c, refComp[c], refThermo.phi[c], logRef[c]);
         }
 
         for (int cand = 0; cand < 3; ++cand)
@@ -1915,6 +1922,7 @@ public:
                 if (err.bad())
                     return result;
                 const Trial t = stabilityTrial(cand, P, T, logRef, refComp, seed);
+                NN_TRACE("trial cand=%d k=%d valid=%d trivial=%d sum=%.17g\n", cand, k, (int)t.valid, (int)t.trivial, t.sum);
                 if (err.bad())
                     return result;
                 if (t.valid && (!best.valid || t.sum > best.sum))
@@ -2023,6 +2031,7 @@ public:
                 return current;
 
             uint8_t fractionPresence = presenceFromFractions(current.beta);
+            NN_TRACE("eq outer=%d active=%d conv=%d it=%d beta=%.17g %.17g %.17g\n", outer, (int)active, (int)current.converged, current.iterations, current.beta[0], current.beta[1], current.beta[2]);
             if (fractionPresence == 0)
                 fractionPresence = chooseSinglePhase(P, T, z);
             if (fractionPresence != active)
@@ -2042,6 +2051,7 @@ public:
                 return current;
             }
             const Stab st = stabilityTest(P, T, z, active, current.x);
+            NN_TRACE("stab active=%d valid=%d stable=%d unst=%d%d%d sums=%.17g %.17g %.17g\n", (int)active, (int)st.valid, (int)st.stable, (int)st.unstable[0], (int)st.unstable[1], (int)st.unstable[2], st.trialSum[0], st.trialSum[1], st.trialSum[2]);
             if (err.bad())
                 return current;
             if (!st.valid)
@@ -2294,6 +2304,7 @@ public:
         initialWaterOilK(kw);
 
         Result seed = flashTwo(P, T, z, kOil, kGas, kg);
+        NN_TRACE("seed conv=%d it=%d beta=%.17g %.17g %.17g\n", (int)seed.converged, seed.iterations, seed.beta[0], seed.beta[1], seed.beta[2]);
         if (err.bad())
             return seed;
         uint8_t active = presenceFromFractions(seed.beta);
@@ -2305,6 +2316,7 @@ public:
                 return seed;
         }
         Result selected = equilibrateActiveSet(P, T, z, active, seed);
+        NN_TRACE("selected conv=%d pres=%d beta=%.17g %.17g %.17g\n", (int)selected.converged, (int)selected.presence, selected.beta[0], selected.beta[1], selected.beta[2]);
         if (err.bad())
             return selected;
         if (selected.converged)
@@ -2515,11 +2527,14 @@ __host__ __device__ FlashResult<NC> Flash<NC>::flashRestrictedImpl(double P, dou
     const int count = popcount3(allowed);
     if (count == 3)
     {
-        // Unreachable from the reference's call sites (restricted flashes are
-        // only requested for reduced sets); evaluated without the nested
-        // recovery stages.
-        bool done = false;
-        return flashMain(P, T, z, done);
+        // The reference forwards an all-phase restriction to the unrestricted
+        // flash (restricted.inc:20-21).  None of its callers reaches this
+        // branch (restricted flashes are requested for reduced sets only), and
+        // the public all-phase request is routed to flash() by the kernel, so
+        // it is rejected here: calling flash() from inside it would make the
+        // device call graph recursive and its stack size unbounded.
+        err.raise(kErrInvalid);
+        return fail;
     }
     if (count == 1)
         return singlePhaseResult(P, T, z, allowed);

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -28,6 +29,7 @@ struct nn_context
     FlashOptions *dOpts = nullptr;
     unsigned int *dCounter = nullptr;  // work-queue head for persistent kernels
     int numCUs = 256;
+    int64_t maxBlocks = 0;  // NN_PERSISTENT_BLOCKS override of the persistent grid (0 = auto)
 };
 
 namespace
@@ -71,7 +73,7 @@ int dispatchNc(int nc, F &&f)
 unsigned int persistentGrid(const nn_context *ctx, int64_t n)
 {
     // 16 wavefront-blocks per CU keeps every SIMD busy while the queue drains.
-    const int64_t cap = int64_t(ctx->numCUs) * 16;
+    const int64_t cap = ctx->maxBlocks > 0 ? ctx->maxBlocks : int64_t(ctx->numCUs) * 16;
     const int64_t need = (n + kBlock - 1) / kBlock;
     return static_cast<unsigned int>(std::max<int64_t>(1, std::min(cap, need)));
 }
@@ -212,6 +214,8 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->numCUs = prop.multiProcessorCount;
+    if (const char *env = std::getenv("NN_PERSISTENT_BLOCKS"))
+        ctx->maxBlocks = std::max(0L, std::atol(env));
     nn_flash_options defaults;
     nn_flash_options_default(&defaults);
     ctx->host = toParams(*fluid);

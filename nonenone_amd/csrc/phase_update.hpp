@@ -349,11 +349,10 @@ struct PhaseUpdater
         return true;
     }
 
-    // :282-496; returns the update flags, status through `status`.
-    __host__ __device__ uint8_t update(double *prim, PhaseStateData<NC> &ps, int &status)
+    // :282-496; returns (status << 8) | update flags.
+    __host__ __device__ int update(double *prim, PhaseStateData<NC> &ps)
     {
-        status = NN_STATUS_OK;
-        uint8_t flags = 0;
+        int flags = 0;
         canonicalizeSwNonAqueousRoles(prim, ps);
         if (err.bad())
             return flags;
@@ -454,8 +453,7 @@ struct PhaseUpdater
             flags |= kFlagUnrestrictedFailed;
             for (int k = 0; k < NP; ++k) prim[k] = original[k];
             ps = originalPs;
-            status = NN_STATUS_RECOVERABLE_FAILURE;
-            return flags;
+            return flags | (NN_STATUS_RECOVERABLE_FAILURE << 8);
         }
         else if (popcount3(active) < 3)
         {
@@ -481,8 +479,7 @@ struct PhaseUpdater
                 flags |= kFlagUnrestrictedFailed;
                 for (int k = 0; k < NP; ++k) prim[k] = original[k];
                 ps = originalPs;
-                status = NN_STATUS_RECOVERABLE_FAILURE;
-                return flags;
+                return flags | (NN_STATUS_RECOVERABLE_FAILURE << 8);
             }
         }
 
@@ -570,8 +567,9 @@ __host__ __device__ void phaseUpdateCell(const FluidParams &f, const FlashOption
     Err err;
     PhaseUpdater<NC, Well> up(f, o, err, T);
     up.sanitize(prim);
-    int st = NN_STATUS_OK;
-    const uint8_t fl = up.update(prim, ps, st);
+    const int packed = up.update(prim, ps);
+    const int st = packed >> 8;
+    const uint8_t fl = uint8_t(packed & 0xff);
     if (err.bad())
     {
         // The reference throws out of updatePhaseState; inputs stay untouched.

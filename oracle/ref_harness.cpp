@@ -463,26 +463,32 @@ struct AssemblyRunner
     static constexpr int NP = I::numPrimaryVariables;
     static constexpr int NPS = I::numPhaseStateVariables;
 
+    // One fluid + kernel per thread: the reference objects hold mutable state.
+    struct Worker
+    {
+        MPMC::FluidSystem<I> fluid;
+        Kernel kernel;
+        explicit Worker(const nn_fluid_desc &d) : fluid(makeFluid<I>(d, 300.0)), kernel(fluid) {}
+    };
+
     const Handle &h;
     const Topology &t;
-    MPMC::FluidSystem<I> fluid;
-    Kernel kernel;
     std::vector<State> states;
     std::vector<Props> props;
 
-    AssemblyRunner(const Handle &hh, const Topology &tt)
-        : h(hh), t(tt), fluid(makeFluid<I>(hh.desc, 300.0)), kernel(fluid)
-    {
-    }
-
-    // natural_petsc_runtime_cell_cache.inc:58-116 (every local block).
-    void buildCache(const double *primary, const double *phaseState)
+    AssemblyRunner(const Handle &hh, const Topology &tt) : h(hh), t(tt)
     {
         states.resize(t.ncell);
         props.resize(t.ncell);
-        for (int64_t c = 0; c < t.ncell; ++c)
+    }
+
+    // natural_petsc_runtime_cell_cache.inc:58-116 for cells [b, e).
+    void buildCache(Worker &w, const double *primary, const double *phaseState, int64_t b,
+                    int64_t e)
+    {
+        for (int64_t c = b; c < e; ++c)
         {
-            fluid.temperature = t.T[c];
+            w.fluid.temperature = t.T[c];
             std::array<double, NP> prim{};
             std::array<double, NPS> ps{};
             for (int k = 0; k < NP; ++k)
@@ -490,18 +496,18 @@ struct AssemblyRunner
             for (int k = 0; k < NPS; ++k)
                 ps[k] = phaseState[c * NPS + k];
             const auto data = MPMC::PetscPhaseStateCodec<I>::decode(ps);
-            states[c] = kernel.decodeState(prim, data);
+            states[c] = w.kernel.decodeState(prim, data);
             std::array<Scalar, NC> overall{};
             for (int k = 0; k < NC; ++k)
                 overall[k] = Scalar(data.overallComposition[k]);
-            props[c] = kernel.evaluateProperties(
+            props[c] = w.kernel.evaluateProperties(
                 states[c], Scalar(t.porosity[c]), 0.0, 2.0, &overall);
         }
     }
 
-    void accumulation(double *accum)
+    void accumulation(double *accum, int64_t b, int64_t e)
     {
-        for (int64_t c = 0; c < t.ncell; ++c)
+        for (int64_t c = b; c < e; ++c)
         {
             const auto scalar = MPMC::scalarizeCellProperties<I>(props[c]);
             const auto a = MPMC::computeFluidAccumulation<I>(scalar);
@@ -510,15 +516,15 @@ struct AssemblyRunner
         }
     }
 
-    void evaluate(double dt, double fugScale, const double *accumPrev,
-                  double *residual, double *bsr, uint8_t *decisionsOut)
+    // residual.inc:59-120 and jacobian.inc:20-237 for cells [b, e); each cell
+    // writes its residual row, its diagonal block and its reciprocal blocks.
+    void evaluate(Worker &w, double dt, double fugScale, const double *accumPrev,
+                  double *residual, double *bsr, uint8_t *decisionsOut, int64_t b, int64_t e)
     {
         constexpr int N = NP;
         std::vector<MPMC::FaceFluxLinearizationDecision<I>> decisions;
         std::vector<double> selfBlock(N * N);
-        if (bsr)
-            std::fill(bsr, bsr + (t.rowOffsets[t.ncell]) * N * N, 0.0);
-        for (int64_t c = 0; c < t.ncell; ++c)
+        for (int64_t c = b; c < e; ++c)
         {
             MPMC::CellProperties<I, double> prevProps{};
             MPMC::AccumulationResult<I, double> prevAccum{};
@@ -527,7 +533,6 @@ struct AssemblyRunner
             const int64_t cb = t.connOffsets[c], ce = t.connOffsets[c + 1];
             decisions.assign(static_cast<std::size_t>(ce - cb), {});
 
-            // residual.inc:59-120: scalar face fluxes record upwind decisions
             MPMC::FaceMassFlux<I, double> outward{};
             for (int64_t k = cb; k < ce; ++k)
             {
@@ -543,7 +548,7 @@ struct AssemblyRunner
                         decisionsOut[k * 3 + p] = static_cast<uint8_t>(
                             decisions[static_cast<std::size_t>(k - cb)].upwind[p]);
             }
-            auto block = kernel.assembleLocalResidual(
+            auto block = w.kernel.assembleLocalResidual(
                 states[c], props[c], prevProps, t.volume[c], dt, fugScale, -1,
                 2650.0, nullptr, &prevAccum, nullptr);
             if (residual)
@@ -557,11 +562,9 @@ struct AssemblyRunner
             }
             if (!bsr)
                 continue;
-            // jacobian.inc:240-271 self block (scaling disabled)
             for (int r = 0; r < N; ++r)
                 for (int q = 0; q < N; ++q)
                     selfBlock[r * N + q] = block.value[r].derivative(q);
-            // jacobian.inc:120-237 face AD pass, reusing residual decisions
             for (int64_t k = cb; k < ce; ++k)
             {
                 const int64_t nb = t.connNeighbor[k];
@@ -582,8 +585,8 @@ struct AssemblyRunner
                 }
             }
             double *self = bsr + findSlot(t, c, c) * N * N;
-            for (int e = 0; e < N * N; ++e)
-                self[e] += selfBlock[e];
+            for (int q = 0; q < N * N; ++q)
+                self[q] += selfBlock[q];
         }
     }
 };
@@ -592,14 +595,25 @@ template <int NC, bool Well>
 int assemblyRun(const Handle &h, const Topology &t, double dt, double fugScale,
                 const double *primary, const double *phaseState,
                 const double *accumPrev, double *residual, double *bsr,
-                uint8_t *decisions, double *accumOut)
+                uint8_t *decisions, double *accumOut, int threads)
 {
-    AssemblyRunner<NC, Well> runner(h, t);
-    runner.buildCache(primary, phaseState);
+    using Runner = AssemblyRunner<NC, Well>;
+    constexpr int N = Runner::NP;
+    Runner runner(h, t);
+    if (bsr)
+        std::fill(bsr, bsr + t.rowOffsets[t.ncell] * N * N, 0.0);
+    // Properties of every cell first (neighbours read them), then the cell loop.
+    parallelFor(t.ncell, threads, [&](int, int64_t b, int64_t e) {
+        typename Runner::Worker w(h.desc);
+        runner.buildCache(w, primary, phaseState, b, e);
+    });
     if (accumOut)
-        runner.accumulation(accumOut);
+        runner.accumulation(accumOut, 0, t.ncell);
     if (residual || bsr)
-        runner.evaluate(dt, fugScale, accumPrev, residual, bsr, decisions);
+        parallelFor(t.ncell, threads, [&](int, int64_t b, int64_t e) {
+            typename Runner::Worker w(h.desc);
+            runner.evaluate(w, dt, fugScale, accumPrev, residual, bsr, decisions, b, e);
+        });
     return NN_OK;
 }
 
@@ -689,7 +703,7 @@ int ref_assembly(void *hv, int64_t ncell, int hasWell, const int64_t *connOffset
                  const double *porosity, const double *T, double dt,
                  double fugScale, const double *primary, const double *phaseState,
                  const double *accumPrev, double *residual, double *bsr,
-                 uint8_t *decisions, double *accumOut)
+                 uint8_t *decisions, double *accumOut, int threads)
 {
     const Handle &h = *static_cast<Handle *>(hv);
     Topology t{ncell, connOffsets, connNeighbor, connTrans, connGdz, rowOffsets,
@@ -700,9 +714,9 @@ int ref_assembly(void *hv, int64_t ncell, int hasWell, const int64_t *connOffset
             constexpr int NC = decltype(nc)::value;
             return hasWell
                 ? assemblyRun<NC, true>(h, t, dt, fugScale, primary, phaseState,
-                                        accumPrev, residual, bsr, decisions, accumOut)
+                                        accumPrev, residual, bsr, decisions, accumOut, threads)
                 : assemblyRun<NC, false>(h, t, dt, fugScale, primary, phaseState,
-                                         accumPrev, residual, bsr, decisions, accumOut);
+                                         accumPrev, residual, bsr, decisions, accumOut, threads);
         });
     }
     catch (const std::exception &e)

@@ -36,7 +36,7 @@ def lib():
         L.ref_stability.argtypes = [_P, _I64] + [_P] * 11
         L.ref_phase_update.argtypes = [_P, _I64, ctypes.c_int] + [_P] * 5 + [ctypes.c_int]
         L.ref_assembly.argtypes = ([_P, _I64, ctypes.c_int] + [_P] * 9 +
-                                   [ctypes.c_double, ctypes.c_double] + [_P] * 7)
+                                   [ctypes.c_double, ctypes.c_double] + [_P] * 7 + [ctypes.c_int])
         _lib = L
     return _lib
 
@@ -120,7 +120,7 @@ class Reference:
         return primary, phase_state, status, flags
 
     def assembly(self, topo, volume, porosity, T, primary, phase_state, accum_prev, dt,
-                 fug_scale=1.0, has_well=False, want_residual=True, want_bsr=True):
+                 fug_scale=1.0, has_well=False, want_residual=True, want_bsr=True, threads=1):
         """Restated runtime loops around the reference's per-cell kernels."""
         n = len(T)
         np_ = primary.shape[1]
@@ -143,7 +143,7 @@ class Reference:
                                  _p(arr["col_blocks"]), _p(arr["volume"]), _p(arr["porosity"]),
                                  _p(T), float(dt), float(fug_scale), _p(primary),
                                  _p(phase_state), _p(accum_prev), _p(res), _p(bsr), _p(dec),
-                                 _p(acc))
+                                 _p(acc), int(threads))
         if rc != 0:
             raise RuntimeError(f"ref_assembly failed: {rc}")
         return res, bsr, dec, acc
