@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Throughput of the per-cell phase update + Natural residual/Jacobian assembly.
+
+One step = one Newton assembly of the whole grid, the work the reference's
+NaturalPetscRuntime does per Newton iteration (runtime state update, cell cache,
+formFunction + formJacobian): every cell goes through the hysteretic phase
+update (stability + flash where the active set changes), AD properties, the
+accumulation/fugacity/closure rows and the 7-point TPFA face fluxes, and its
+block row of the BSR Jacobian is written in the cached pattern order.
+
+Workload: 200 x 200 x 50 structured grid (2,000,000 cells, dx = dy = 10 m,
+dz = 5 m), nc = 3 H2O/CO2/nC10, PR78 with k(H2O,CO2) = 0.19 and
+k(H2O,nC10) = 0.5, a well unknown per cell (11 primary variables, 11x11 BSR
+blocks), hydrostatic pressure + P ~ U(8, 30) MPa, T ~ U(320, 450) K,
+z ~ Dirichlet(4, 1, 1), porosity ~ U(0.1, 0.3), log-normal permeability
+(median 100 mD).  History state = flash(P, T, z) written back as the primary
+and phase-state vectors; the assembled state is that history plus a seeded
+Newton-like perturbation (5 % of cells get an active saturation pushed below
+zero, which drives the phase-removal / stability path).  Inputs are resident
+in HBM before the timed region; each step restores the perturbed state with a
+device copy (part of the timed work) because the phase update is in place.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+
+Prints one JSON line.  The `cpu_baseline` leg runs the reference's own code
+(the oracle's build of it, oracle/_ref) on a 50 x 50 x 50 sub-block of the
+same workload with one thread per host core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "cell flash+Jacobian assemblies/s (FP64, ≤1e-10 rel); p50/p99 batch latency"
+HBM_PEAK_GBS = 8000.0
+DT = 86400.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def assemble_bytes(ncell: int, nconn: int, nnzb: int, nc: int, np_: int, nps: int) -> int:
+    """Compulsory HBM bytes of one assembly launch: read primary, phase state,
+    T, porosity, volume and M^n per cell and (T_ij, g dz) per connection; write
+    the residual row and every BSR block once."""
+    read = ncell * 8 * (np_ + nps + 3 + nc) + nconn * 16
+    write = ncell * 8 * np_ + nnzb * 8 * np_ * np_
+    return read + write
+
+
+def build_workload(nx, ny, nz, seed=0):
+    import torch
+
+    from nonenone_amd import fluids, workloads
+    from nonenone_amd.assembly import NaturalAssembly, assign_flash_result, perturb_newton_state
+    from nonenone_amd.flash import ThreePhaseFlash
+
+    fluid = fluids.h2o_co2_nc10("PR")
+    nc = fluid.nc
+    t0 = time.time()
+    w = workloads.grid_workload(nx, ny, nz, 10.0, 10.0, 5.0, fluid, seed=seed, hydrostatic=True)
+    log(f"grid {nx}x{ny}x{nz}: {w['ncell']} cells, {int(w['topo']['conn_offsets'][-1])} "
+        f"connections, {int(w['topo']['row_offsets'][-1])} blocks ({time.time() - t0:.1f}s)")
+    fl = ThreePhaseFlash(fluid)
+    t0 = time.time()
+    keys = ("presence", "status", "beta", "saturation", "x", "Z", "k_go", "k_wo")
+    parts = {k: [] for k in keys}
+    chunk = 1 << 18
+    for a in range(0, w["ncell"], chunk):
+        b = min(a + chunk, w["ncell"])
+        res = fl.flash(w["P"][a:b], w["T"][a:b], w["z"][a:b])
+        for k in keys:
+            parts[k].append(getattr(res, k).cpu().numpy())
+        log(f"history flash {b}/{w['ncell']} ({time.time() - t0:.1f}s)")
+    flash = {k: np.concatenate(v) for k, v in parts.items()}
+    del parts, res
+    counts = np.bincount(flash["presence"].astype(np.int64), minlength=8)
+    log(f"history flash of {w['ncell']} states: {time.time() - t0:.1f}s; presence counts "
+        f"{counts.tolist()}; non-OK statuses {int((flash['status'] != 0).sum())}")
+    primary, ps = assign_flash_result(flash, w["P"], has_well=True)
+    pert = perturb_newton_state(primary, ps, nc, seed=seed + 1)
+    dev = torch.device("cuda")
+    asm = NaturalAssembly(fl, w["topo"], w["volume"], w["porosity"], w["T"], has_well_unknown=True)
+    hist_p = torch.as_tensor(primary, device=dev)
+    hist_s = torch.as_tensor(ps, device=dev)
+    acc = asm.accumulation(hist_p, hist_s)
+    del hist_p, hist_s
+    state = dict(prim_pert=torch.as_tensor(pert, device=dev), ps_pert=torch.as_tensor(ps, device=dev),
+                 acc=acc, T=torch.as_tensor(w["T"], device=dev))
+    state["prim"] = state["prim_pert"].clone()
+    state["ps"] = state["ps_pert"].clone()
+    state["out"] = asm.allocate(dev)
+    host = dict(pert=pert, ps=ps, w=w, acc=acc.cpu().numpy())
+    return fluid, fl, asm, state, host
+
+
+def run_step(fl, asm, s):
+    s["prim"].copy_(s["prim_pert"])
+    s["ps"].copy_(s["ps_pert"])
+    st, _ = fl.update_phase_state(s["T"], s["prim"], s["ps"], has_well_unknown=True)
+    asm.evaluate(DT, s["prim"], s["ps"], s["acc"], out=s["out"])
+    return st
+
+
+def cpu_baseline(fluid, host, dims, sub=(50, 50, 50), target_s=12.0):
+    """The reference's phase update + assembly (oracle/_ref build) on a sub-block."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import refharness
+
+    from nonenone_amd import workloads
+    from nonenone_amd._abi import FlashOptions
+
+    if not refharness.available():
+        log("oracle/_ref/libnn_ref.so not built: no CPU baseline")
+        return None
+    w = host["w"]
+    nx, ny, nz = dims
+    sx, sy, sz = (min(a, b) for a, b in zip(sub, dims))
+    ii, jj, kk = np.meshgrid(np.arange(sx), np.arange(sy), np.arange(sz), indexing="ij")
+    cells = ((kk * ny + jj) * nx + ii).transpose(2, 1, 0).reshape(-1)  # natural order of the sub-block
+    topo = workloads.structured_topology(sx, sy, sz, 10.0, 10.0, 5.0, w["perm"][cells])
+    ref = refharness.Reference(fluid, FlashOptions.defaults())
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    T = w["T"][cells]
+    prim0 = host["pert"][cells]
+    ps0 = host["ps"][cells]
+    acc_prev = host["acc"][cells]
+    t_total = 0.0
+    passes = 0
+    while t_total < target_s and passes < 50:
+        t0 = time.perf_counter()
+        prim, ps, _, _ = ref.phase_update(T, prim0, ps0, has_well=True, threads=cores)
+        ref.assembly(topo, w["volume"][cells], w["porosity"][cells], T, prim, ps, acc_prev, DT,
+                     has_well=True, threads=cores)
+        t_total += time.perf_counter() - t0
+        passes += 1
+    n = len(cells)
+    return {"value": n * passes / t_total, "unit": "cells/s", "cores": cores, "kind": "reference",
+            "sample": f"{sx}x{sy}x{sz} sub-block ({n} cells) of the same workload, {passes} passes "
+                      f"of phase update + residual + BSR Jacobian in {t_total:.1f}s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=200)
+    ap.add_argument("--ny", type=int, default=200)
+    ap.add_argument("--nz", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the path is per-cell and shards by slabs of the grid "
+              "with a halo exchange that this build does not implement; run with --gpus 1.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 1
+    fluid, fl, asm, s, host = build_workload(args.nx, args.ny, args.nz)
+    ncell, nconn, nnzb = asm.ncell, asm.nconn, asm.nnzb
+    for i in range(args.warmup):
+        t0 = time.time()
+        st = run_step(fl, asm, s)
+        torch.cuda.synchronize()
+        log(f"warmup {i}: {time.time() - t0:.3f}s; phase-update non-OK "
+            f"{int((st != 0).sum().item())}; assembly non-OK "
+            f"{int((s['out']['status'] != 0).sum().item())}")
+
+    fl.set_kernel_timing(True)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record()
+        run_step(fl, asm, s)
+        ends[i].record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
+    kern = {}
+    for kind in ("phase_update", "cell_record", "assemble"):
+        ms, count = fl.kernel_time(kind)
+        kern[kind] = dict(total_ms=ms, launches=count, avg_ms=ms / max(count, 1))
+    fl.set_kernel_timing(False)
+    log(f"timed {args.steps} steps in {elapsed:.3f}s; kernels "
+        + ", ".join(f"{k} {v['avg_ms']:.2f} ms" for k, v in kern.items()))
+
+    nc = fluid.nc
+    np_, nps = asm.np_, asm.nps
+    abytes = assemble_bytes(ncell, nconn, nnzb, nc, np_, nps)
+    avg_s = kern["assemble"]["avg_ms"] * 1e-3
+    achieved = abytes / avg_s / 1e9
+    roofline = {"kernel": "assembleKernel<3,true>", "bound": "hbm", "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None, "algorithmic_bytes": abytes}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        cpu = cpu_baseline(fluid, host, (args.nx, args.ny, args.nz))
+        log(f"cpu baseline {time.time() - t0:.1f}s: {cpu}")
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    line = {
+        "metric": METRIC, "value": ncell * args.steps / elapsed, "unit": "cells/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"natural assembly {args.nx}x{args.ny}x{args.nz} "
+                               f"({ncell} cells), nc=3 H2O/CO2/nC10 PR78, 7-point TPFA, "
+                               f"well unknown per cell (N_p=11), phase update + residual + "
+                               f"BSR Jacobian ({nnzb} 11x11 blocks)",
+                   "eos": "PR78 (PR-CPA not in this build)", "cells": ncell,
+                   "connections": nconn, "bsr_blocks": nnzb, "n_primary": np_},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "kernels_ms": {k: v["avg_ms"] for k, v in kern.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

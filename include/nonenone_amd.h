@@ -123,6 +123,25 @@ int nn_context_destroy(nn_context *ctx);
 const char *nn_last_error(void);
 
 /*
+ * Kernel timing (no counterpart in the reference: instrumentation for
+ * throughput and roofline figures).  While enabled, every launch made through
+ * this context is bracketed by HIP events recorded on the launch's own stream;
+ * nn_context_kernel_time waits for those events and returns the summed
+ * duration and launch count of one kernel kind since timing was last enabled.
+ * Enabling again clears the record.
+ */
+#define NN_KERNEL_EOS 0
+#define NN_KERNEL_FLASH 1
+#define NN_KERNEL_STABILITY 2
+#define NN_KERNEL_PHASE_UPDATE 3
+#define NN_KERNEL_CELL_RECORD 4
+#define NN_KERNEL_ASSEMBLE 5
+#define NN_KERNEL_KINDS 6
+int nn_context_set_kernel_timing(nn_context *ctx, int32_t enable);
+int nn_context_kernel_time(nn_context *ctx, int32_t kind, double *total_ms,
+                           int64_t *launches);
+
+/*
  * E1-E5: batched single-phase EOS evaluation, CubicEquationOfState::phaseResult
  * with a phase role (cubic_eos.hpp:834-856): mixingParameters -> phaseMixing ->
  * liquid/vapour root -> fugacityAtRoot.  role[i] in {0 oil,1 gas,2 water}.

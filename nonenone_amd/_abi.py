@@ -158,7 +158,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.nn_assembly_destroy.argtypes = [_P]
     L.nn_assembly_accumulation.argtypes = [_P, _P, _P, _P, _P]
     L.nn_assembly_evaluate.argtypes = [_P, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P]
-    for name in ("nn_context_create", "nn_context_destroy", "nn_eos_phase_batch",
+    L.nn_context_set_kernel_timing.argtypes = [_P, _I32]
+    L.nn_context_kernel_time.argtypes = [_P, _I32, ctypes.POINTER(_D), ctypes.POINTER(_I64)]
+    for name in ("nn_context_set_kernel_timing", "nn_context_kernel_time","nn_context_create", "nn_context_destroy", "nn_eos_phase_batch",
                  "nn_flash_batch", "nn_flash_restricted_batch", "nn_stability_batch",
                  "nn_phase_update_batch", "nn_structured_topology", "nn_assembly_create",
                  "nn_assembly_destroy", "nn_assembly_accumulation", "nn_assembly_evaluate"):
@@ -171,8 +173,12 @@ EXPORTED_SYMBOLS = (
     "nn_eos_phase_batch", "nn_flash_batch", "nn_flash_restricted_batch",
     "nn_stability_batch", "nn_phase_update_batch", "nn_structured_topology",
     "nn_assembly_create", "nn_assembly_destroy", "nn_assembly_accumulation",
-    "nn_assembly_evaluate",
+    "nn_assembly_evaluate", "nn_context_set_kernel_timing", "nn_context_kernel_time",
 )
+
+# Kernel kinds of nn_context_kernel_time.
+KERNEL_KINDS = {"eos": 0, "flash": 1, "stability": 2, "phase_update": 3, "cell_record": 4,
+                "assemble": 5}
 
 
 def check(rc: int, what: str) -> None:

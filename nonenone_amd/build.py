@@ -63,8 +63,7 @@ def _compile(unit: str) -> tuple[str, Path, str]:
         if "Dynamic Stack: True" in line:
             raise RuntimeError(f"{unit}: kernel {kernel} has a dynamic (unbounded) stack")
     warnings = "\n".join(line for line in _filtered(proc.stderr).splitlines()
-                         if "kernel-resource-usage" not in line and not line.lstrip().startswith(("|", "^"))
-                         and not line.strip().isdigit())
+                         if "warning:" in line)
     return unit, obj, warnings
 
 

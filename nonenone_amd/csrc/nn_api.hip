@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "../../include/nonenone_amd.h"
@@ -30,6 +31,9 @@ struct nn_context
     unsigned int *dCounter = nullptr;  // work-queue head for persistent kernels
     int numCUs = 256;
     int64_t maxBlocks = 0;  // NN_PERSISTENT_BLOCKS override of the persistent grid (0 = auto)
+    // Kernel timing: (start, end) event pairs per kernel kind while enabled.
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events[NN_KERNEL_KINDS];
 };
 
 namespace
@@ -69,6 +73,53 @@ int dispatchNc(int nc, F &&f)
     }
 }
 
+
+// Work-queue head reset by a kernel rather than hipMemsetAsync, so that the
+// launch sequence stays valid inside a captured HIP graph.
+__global__ void zeroCounterKernel(unsigned int *counter) { *counter = 0u; }
+
+void releaseEvents(nn_context *ctx)
+{
+    for (auto &list : ctx->events)
+    {
+        for (auto &pr : list)
+        {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        list.clear();
+    }
+}
+
+// Brackets one launch with events on its stream when the context is timing.
+struct LaunchTimer
+{
+    nn_context *ctx;
+    int kind;
+    hipStream_t st;
+    hipEvent_t start = nullptr;
+    LaunchTimer(nn_context *c, int k, hipStream_t s) : ctx(c), kind(k), st(s)
+    {
+        if (ctx->timing && hipEventCreate(&start) == hipSuccess)
+            (void)hipEventRecord(start, st);
+    }
+    void done()
+    {
+        if (!start)
+            return;
+        hipEvent_t end = nullptr;
+        if (hipEventCreate(&end) == hipSuccess && hipEventRecord(end, st) == hipSuccess)
+            ctx->events[kind].emplace_back(start, end);
+        else
+            (void)hipEventDestroy(start);
+        start = nullptr;
+    }
+    ~LaunchTimer()
+    {
+        if (start)
+            (void)hipEventDestroy(start);
+    }
+};
 
 unsigned int persistentGrid(const nn_context *ctx, int64_t n)
 {
@@ -233,10 +284,38 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
     return NN_OK;
 }
 
+int nn_context_set_kernel_timing(nn_context *ctx, int32_t enable)
+{
+    if (!ctx)
+        return fail(NN_E_INVALID_ARGUMENT, "null context");
+    NN_HIP(hipDeviceSynchronize());
+    releaseEvents(ctx);
+    ctx->timing = enable != 0;
+    return NN_OK;
+}
+
+int nn_context_kernel_time(nn_context *ctx, int32_t kind, double *total_ms, int64_t *launches)
+{
+    if (!ctx || kind < 0 || kind >= NN_KERNEL_KINDS || !total_ms || !launches)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    double sum = 0.0;
+    for (auto &pr : ctx->events[kind])
+    {
+        NN_HIP(hipEventSynchronize(pr.second));
+        float ms = 0.0f;
+        NN_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = int64_t(ctx->events[kind].size());
+    return NN_OK;
+}
+
 int nn_context_destroy(nn_context *ctx)
 {
     if (!ctx)
         return NN_OK;
+    releaseEvents(ctx);
     if (ctx->dFluid) (void)hipFree(ctx->dFluid);
     if (ctx->dOpts) (void)hipFree(ctx->dOpts);
     if (ctx->dCounter) (void)hipFree(ctx->dCounter);
@@ -258,8 +337,10 @@ int nn_eos_phase_batch(nn_context *ctx, int64_t n, const double *d_P, const doub
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
         const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+        LaunchTimer t(ctx, NN_KERNEL_EOS, st);
         NN_HIP((Launch<NC>::eos(grid, st, ctx->dFluid, n, d_P, d_T, d_x, d_role, d_Z, d_lnphi,
                                 d_fug, d_status)));
+        t.done();
         return NN_OK;
     });
 }
@@ -282,9 +363,12 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
     const nn_flash_out o = *out;
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        NN_HIP(hipMemsetAsync(ctx->dCounter, 0, sizeof(unsigned int), st));
+        zeroCounterKernel<<<1, 1, 0, st>>>(ctx->dCounter);
+        NN_HIP(hipGetLastError());
+        LaunchTimer t(ctx, NN_KERNEL_FLASH, st);
         NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), st, ctx->dFluid, ctx->dOpts, n, d_P, d_T,
                                   d_z, d_allowed, d_seed, o, ctx->dCounter)));
+        t.done();
         return NN_OK;
     });
 }
@@ -320,9 +404,11 @@ int nn_stability_batch(nn_context *ctx, int64_t n, const double *d_P, const doub
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
         const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+        LaunchTimer t(ctx, NN_KERNEL_STABILITY, st);
         NN_HIP((Launch<NC>::stability(grid, st, ctx->dFluid, ctx->dOpts, n, d_P, d_T, d_z, d_active,
                                       d_x, d_stable, d_valid, d_unstable_mask, d_trial_sum,
                                       d_incipient, d_status)));
+        t.done();
         return NN_OK;
     });
 }
@@ -344,10 +430,13 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
     hipStream_t st = static_cast<hipStream_t>(stream);
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        NN_HIP(hipMemsetAsync(ctx->dCounter, 0, sizeof(unsigned int), st));
+        zeroCounterKernel<<<1, 1, 0, st>>>(ctx->dCounter);
+        NN_HIP(hipGetLastError());
+        LaunchTimer t(ctx, NN_KERNEL_PHASE_UPDATE, st);
         NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, persistentGrid(ctx, n), st,
                                         ctx->dFluid, ctx->dOpts, n, d_T, d_primary, d_phase_state,
                                         d_status, d_flags, ctx->dCounter)));
+        t.done();
         return NN_OK;
     });
 }
@@ -560,11 +649,15 @@ int nn_assembly_evaluate(nn_assembly *a, double dt, double fugacity_scaling,
     const AssemblyView v = a->view;
     return dispatchNc(a->ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
+        LaunchTimer tr(a->ctx, NN_KERNEL_CELL_RECORD, st);
         NN_HIP((Launch<NC>::record(a->hasWell != 0, st, a->ctx->dFluid, v, d_primary, d_phase_state,
                                    nullptr, nullptr)));
+        tr.done();
+        LaunchTimer ta(a->ctx, NN_KERNEL_ASSEMBLE, st);
         NN_HIP((Launch<NC>::assemble(a->hasWell != 0, st, a->ctx->dFluid, v, dt, fugacity_scaling,
                                      d_primary, d_phase_state, d_accum_prev, d_residual, d_bsr,
                                      d_decisions, d_status)));
+        ta.done();
         return NN_OK;
     });
 }

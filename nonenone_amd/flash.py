@@ -96,6 +96,20 @@ class Context:
         except Exception:
             pass
 
+    def set_kernel_timing(self, enable: bool) -> None:
+        """Bracket every launch with HIP events on its stream (clears earlier records)."""
+        _abi.check(self.lib.nn_context_set_kernel_timing(self.handle, int(bool(enable))),
+                   "nn_context_set_kernel_timing")
+
+    def kernel_time(self, kind: str) -> tuple[float, int]:
+        """(summed milliseconds, launches) of one kernel kind since timing was enabled."""
+        ms = ctypes.c_double()
+        count = ctypes.c_int64()
+        _abi.check(self.lib.nn_context_kernel_time(self.handle, _abi.KERNEL_KINDS[kind],
+                                                   ctypes.byref(ms), ctypes.byref(count)),
+                   "nn_context_kernel_time")
+        return ms.value, count.value
+
 
 class ThreePhaseFlash(Context):
     """Batched CubicThreePhaseFlash (PR / PR78 / Soreide-Whitson)."""
