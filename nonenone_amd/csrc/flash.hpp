@@ -2287,6 +2287,567 @@ public:
         return n;
     }
 
+    // ------------------------------------------- SW exact-mass allocation
+    // allocation.inc:8-700.  Unknowns u = (u_g[NC], u_w[NC]) are per-component
+    // softmax logits of the split of z_i over (oil, gas, water), so the
+    // material balance holds exactly for every u; the search minimises Gibbs
+    // energy (BFGS) and then drives the chemical-potential differences to the
+    // fugacity tolerance (Levenberg-Marquardt on a central-difference Jacobian).
+    static constexpr int kAllocDim = 2 * NC;
+
+    struct Allocation
+    {
+        bool valid;
+        double gibbs;
+        double closure;
+        double norm2;
+        double residual[kAllocDim];
+        double gradient[kAllocDim];
+        double x[3][NC];
+        double beta[3];
+        double Z[3];
+    };
+
+    // allocation.inc:32-128
+    __host__ __device__ void evaluateAllocation(double P, double T, const double *z,
+                                                const double *u, Allocation &s)
+    {
+        s.valid = false;
+        s.gibbs = 0.0;
+        s.closure = __builtin_inf();
+        s.norm2 = __builtin_inf();
+        for (int k = 0; k < kAllocDim; ++k)
+        {
+            s.residual[k] = 0.0;
+            s.gradient[k] = 0.0;
+        }
+        double amount[3][NC];
+        for (int p = 0; p < 3; ++p)
+        {
+            s.beta[p] = 0.0;
+            s.Z[p] = 0.0;
+            for (int c = 0; c < NC; ++c) s.x[p][c] = 0.0;
+        }
+        for (int c = 0; c < NC; ++c)
+        {
+            const double ug = u[c];
+            const double uw = u[NC + c];
+            double shift = 0.0;
+            if (shift < ug) shift = ug;
+            if (shift < uw) shift = uw;
+            const double qo0 = ::exp(-shift);
+            const double qg0 = ::exp(ug - shift);
+            const double qw0 = ::exp(uw - shift);
+            const double qsum = qo0 + qg0 + qw0;
+            amount[0][c] = z[c] * (qo0 / qsum);
+            amount[1][c] = z[c] * (qg0 / qsum);
+            amount[2][c] = z[c] * (qw0 / qsum);
+            s.beta[0] += amount[0][c];
+            s.beta[1] += amount[1][c];
+            s.beta[2] += amount[2][c];
+        }
+        for (int p = 0; p < 3; ++p)
+        {
+            if (!(s.beta[p] > 1.0e-14))
+                return;
+            for (int c = 0; c < NC; ++c) s.x[p][c] = amount[p][c] / s.beta[p];
+        }
+        double mu[3][NC];
+        for (int p = 0; p < 3; ++p)
+        {
+            PhaseThermo<NC> t;
+            thermo(p, P, T, s.x[p], t);
+            if (err.bad())
+                return;
+            s.Z[p] = t.Z;
+            for (int c = 0; c < NC; ++c)
+            {
+                const double xc = maxd(s.x[p][c], o.floor);
+                const double phi = maxd(t.phi[c], 1.0e-300);
+                mu[p][c] = ::log(xc) + ::log(phi);
+                s.gibbs += amount[p][c] * mu[p][c];
+            }
+        }
+        s.closure = 0.0;
+        s.norm2 = 0.0;
+        for (int c = 0; c < NC; ++c)
+        {
+            const int w = NC + c;
+            if (!(z[c] > o.floor))
+                continue;  // absent component: zero residual and gradient rows
+            s.residual[c] = mu[1][c] - mu[0][c];
+            s.residual[w] = mu[2][c] - mu[0][c];
+            s.norm2 += s.residual[c] * s.residual[c] + s.residual[w] * s.residual[w];
+            s.closure = maxd(s.closure, ::fabs(mu[0][c] - mu[1][c]));
+            s.closure = maxd(s.closure, ::fabs(mu[0][c] - mu[2][c]));
+            s.closure = maxd(s.closure, ::fabs(mu[1][c] - mu[2][c]));
+            const double qo = amount[0][c] / z[c];
+            const double qg = amount[1][c] / z[c];
+            const double qw = amount[2][c] / z[c];
+            const double average = qo * mu[0][c] + qg * mu[1][c] + qw * mu[2][c];
+            s.gradient[c] = z[c] * qg * (mu[1][c] - average);
+            s.gradient[w] = z[c] * qw * (mu[2][c] - average);
+        }
+        s.valid = finite(s.gibbs) && finite(s.norm2) && finite(s.closure);
+    }
+
+    // allocation.inc:135-238: BFGS on the Gibbs energy (inverse-Hessian form),
+    // Armijo backtracking, steps capped at 4 in the max norm.
+    __host__ __device__ void gibbsPreconditionAllocation(double P, double T, const double *z,
+                                                         double *u, Allocation &state)
+    {
+        double H[kAllocDim][kAllocDim];
+        for (int i = 0; i < kAllocDim; ++i)
+            for (int j = 0; j < kAllocDim; ++j) H[i][j] = (i == j) ? 1.0 : 0.0;
+        evaluateAllocation(P, T, z, u, state);
+        if (err.bad() || !state.valid)
+            return;
+        Allocation trial;
+        for (int iteration = 0; iteration < 160; ++iteration)
+        {
+            double dir[kAllocDim];
+            for (int i = 0; i < kAllocDim; ++i)
+            {
+                dir[i] = 0.0;
+                for (int j = 0; j < kAllocDim; ++j) dir[i] -= H[i][j] * state.gradient[j];
+            }
+            double dd = 0.0, maxStep = 0.0;
+            for (int i = 0; i < kAllocDim; ++i)
+            {
+                dd += state.gradient[i] * dir[i];
+                maxStep = maxd(maxStep, ::fabs(dir[i]));
+            }
+            if (!(dd < 0.0) || !finite(dd))
+            {
+                dd = 0.0;
+                maxStep = 0.0;
+                for (int i = 0; i < kAllocDim; ++i)
+                {
+                    for (int j = 0; j < kAllocDim; ++j) H[i][j] = (i == j) ? 1.0 : 0.0;
+                    dir[i] = -state.gradient[i];
+                    dd += state.gradient[i] * dir[i];
+                    maxStep = maxd(maxStep, ::fabs(dir[i]));
+                }
+            }
+            if (maxStep > 4.0)
+                for (int i = 0; i < kAllocDim; ++i) dir[i] *= 4.0 / maxStep;
+
+            double trialU[kAllocDim];
+            double step = 1.0;
+            bool accepted = false;
+            for (int ls = 0; ls < 24; ++ls)
+            {
+                for (int i = 0; i < kAllocDim; ++i)
+                    trialU[i] = clampd(u[i] + step * dir[i], -40.0, 40.0);
+                evaluateAllocation(P, T, z, trialU, trial);
+                if (err.bad())
+                    return;
+                if (trial.valid && trial.gibbs <= state.gibbs + 1.0e-4 * step * dd)
+                {
+                    accepted = true;
+                    break;
+                }
+                step *= 0.5;
+            }
+            if (!accepted)
+                return;
+
+            double sv[kAllocDim], yv[kAllocDim], hy[kAllocDim];
+            double ys = 0.0;
+            for (int i = 0; i < kAllocDim; ++i)
+            {
+                sv[i] = trialU[i] - u[i];
+                yv[i] = trial.gradient[i] - state.gradient[i];
+            }
+            for (int i = 0; i < kAllocDim; ++i) ys += yv[i] * sv[i];
+            if (ys > 1.0e-12)
+            {
+                double yhy = 0.0;
+                for (int i = 0; i < kAllocDim; ++i)
+                {
+                    hy[i] = 0.0;
+                    for (int j = 0; j < kAllocDim; ++j) hy[i] += H[i][j] * yv[j];
+                }
+                for (int i = 0; i < kAllocDim; ++i) yhy += yv[i] * hy[i];
+                const double coefficient = (ys + yhy) / (ys * ys);
+                for (int i = 0; i < kAllocDim; ++i)
+                    for (int j = 0; j < kAllocDim; ++j)
+                        H[i][j] += coefficient * sv[i] * sv[j] - (hy[i] * sv[j] + sv[i] * hy[j]) / ys;
+            }
+            else
+            {
+                for (int i = 0; i < kAllocDim; ++i)
+                    for (int j = 0; j < kAllocDim; ++j) H[i][j] = (i == j) ? 1.0 : 0.0;
+            }
+            for (int i = 0; i < kAllocDim; ++i) u[i] = trialU[i];
+            state = trial;
+        }
+    }
+
+    // allocation.inc:245-343: Levenberg-Marquardt on (mu_g - mu_o, mu_w - mu_o).
+    __host__ __device__ bool refineAllocationChemicalPotentials(double P, double T, const double *z,
+                                                                double *u, Allocation &state)
+    {
+        evaluateAllocation(P, T, z, u, state);
+        if (err.bad() || !state.valid)
+            return false;
+        double lambda = 1.0e-5;
+        Allocation ps, ms;
+        for (int iteration = 0; iteration < 160; ++iteration)
+        {
+            if (state.closure < o.fugTol)
+                return true;
+            double J[kAllocDim][kAllocDim];
+            for (int col = 0; col < kAllocDim; ++col)
+            {
+                const double h = 2.0e-5 * maxd(1.0, ::fabs(u[col]));
+                double plus[kAllocDim], minus[kAllocDim];
+                for (int i = 0; i < kAllocDim; ++i) plus[i] = minus[i] = u[i];
+                plus[col] = mind(40.0, u[col] + h);
+                minus[col] = maxd(-40.0, u[col] - h);
+                evaluateAllocation(P, T, z, plus, ps);
+                if (err.bad())
+                    return false;
+                evaluateAllocation(P, T, z, minus, ms);
+                if (err.bad())
+                    return false;
+                if (!ps.valid || !ms.valid)
+                    return false;
+                const double den = plus[col] - minus[col];
+                for (int row = 0; row < kAllocDim; ++row)
+                    J[row][col] = (ps.residual[row] - ms.residual[row]) / den;
+            }
+            bool accepted = false;
+            for (int lm = 0; lm < 14 && !accepted; ++lm)
+            {
+                double normal[kAllocDim][kAllocDim], rhs[kAllocDim], delta[kAllocDim];
+                for (int a = 0; a < kAllocDim; ++a)
+                {
+                    for (int b = 0; b < kAllocDim; ++b)
+                    {
+                        double value = 0.0;
+                        for (int row = 0; row < kAllocDim; ++row) value += J[row][a] * J[row][b];
+                        normal[a][b] = value;
+                    }
+                    double value = 0.0;
+                    for (int row = 0; row < kAllocDim; ++row) value += J[row][a] * state.residual[row];
+                    rhs[a] = -value;
+                    normal[a][a] += lambda;
+                }
+                if (!solveDense<kAllocDim>(normal, rhs, delta))
+                {
+                    lambda *= 10.0;
+                    continue;
+                }
+                double maxDelta = 0.0;
+                for (int i = 0; i < kAllocDim; ++i) maxDelta = maxd(maxDelta, ::fabs(delta[i]));
+                if (maxDelta > 6.0)
+                    for (int i = 0; i < kAllocDim; ++i) delta[i] *= 6.0 / maxDelta;
+                double damping = 1.0;
+                for (int ls = 0; ls < 22; ++ls)
+                {
+                    double trialU[kAllocDim];
+                    for (int i = 0; i < kAllocDim; ++i)
+                        trialU[i] = clampd(u[i] + damping * delta[i], -40.0, 40.0);
+                    evaluateAllocation(P, T, z, trialU, ps);
+                    if (err.bad())
+                        return false;
+                    if (ps.valid && ps.norm2 < state.norm2 * (1.0 - 1.0e-5 * damping))
+                    {
+                        for (int i = 0; i < kAllocDim; ++i) u[i] = trialU[i];
+                        state = ps;
+                        lambda = maxd(1.0e-14, lambda * 0.2);
+                        accepted = true;
+                        break;
+                    }
+                    damping *= 0.5;
+                }
+                if (!accepted)
+                    lambda *= 10.0;
+            }
+            if (!accepted)
+                return false;
+        }
+        return state.closure < o.fugTol;
+    }
+
+    // u from per-phase component amounts (the common tail of the seed builders).
+    __host__ __device__ static void logitsFromAmounts(const double *z, const double (*amount)[NC],
+                                                      int c, double *u)
+    {
+        const double qo = maxd(amount[0][c] / z[c], 1.0e-18);
+        const double qg = maxd(amount[1][c] / z[c], 1.0e-18);
+        const double qw = maxd(amount[2][c] / z[c], 1.0e-18);
+        u[c] = ::log(qg / qo);
+        u[NC + c] = ::log(qw / qo);
+    }
+
+    // allocation.inc:359-443
+    __host__ __device__ bool allocationSeedFromPair(double P, double T, const double *z,
+                                                    uint8_t allowed, double *u)
+    {
+        const Result pair = flashRestrictedRecovery(P, T, z, allowed);
+        if (err.bad() || !pair.converged || popcount3(pair.presence) != 2)
+            return false;
+        const Stab st = stabilityTest(P, T, z, pair.presence, pair.x);
+        if (err.bad())
+            return false;
+        int missing = -1;
+        for (int p = 0; p < 3; ++p)
+            if (!(pair.presence & (1u << p)))
+            {
+                missing = p;
+                break;
+            }
+        if (missing < 0 || !st.valid || !st.unstable[missing])
+            return false;
+        const double *inc = st.incipient[missing];
+        double eps = 1.0e-3;
+        for (int c = 0; c < NC; ++c)
+            if (inc[c] > 1.0e-30)
+                eps = mind(eps, 0.15 * z[c] / inc[c]);
+        eps = maxd(eps, 1.0e-9);
+        double amount[3][NC];
+        for (int p = 0; p < 3; ++p)
+            for (int c = 0; c < NC; ++c) amount[p][c] = 0.0;
+        for (int c = 0; c < NC; ++c)
+        {
+            if (!(z[c] > o.floor))
+                continue;
+            amount[missing][c] = eps * inc[c];
+            const double remaining = z[c] - amount[missing][c];
+            if (!(remaining > 0.0))
+                return false;
+            double pairAmount = 0.0;
+            for (int p = 0; p < 3; ++p)
+                if (p != missing)
+                    pairAmount += pair.beta[p] * pair.x[p][c];
+            if (!(pairAmount > 0.0))
+                return false;
+            for (int p = 0; p < 3; ++p)
+                if (p != missing)
+                    amount[p][c] = pair.beta[p] * pair.x[p][c] * remaining / pairAmount;
+        }
+        for (int c = 0; c < NC; ++c)
+        {
+            if (!(z[c] > o.floor))
+            {
+                u[c] = 0.0;
+                u[NC + c] = 0.0;
+                continue;
+            }
+            logitsFromAmounts(z, amount, c, u);
+        }
+        return true;
+    }
+
+    // allocation.inc:446-501
+    __host__ __device__ bool allocationSeedFromRawOg(double P, double T, const double *z, double *u)
+    {
+        double K[NC];
+        wilsonK(P, T, K);
+        Result pair = flashTwo(P, T, z, kOil, kGas, K);
+        if (err.bad() || !pair.converged || popcount3(presenceFromFractions(pair.beta)) != 2)
+            return false;
+        pair.presence = presenceFromFractions(pair.beta);
+        const Stab st = stabilityTest(P, T, z, pair.presence, pair.x);
+        if (err.bad())
+            return false;
+        if (!st.valid || !st.unstable[kWater])
+            return false;
+        const double *inc = st.incipient[kWater];
+        double eps = 1.0e-3;
+        for (int c = 0; c < NC; ++c)
+            if (inc[c] > 1.0e-30)
+                eps = mind(eps, 0.2 * z[c] / inc[c]);
+        eps = maxd(eps, 1.0e-9);
+        double amount[3][NC];
+        for (int c = 0; c < NC; ++c)
+        {
+            if (!(z[c] > o.floor))
+            {
+                u[c] = 0.0;
+                u[NC + c] = 0.0;
+                continue;
+            }
+            amount[2][c] = eps * inc[c];
+            const double remaining = z[c] - amount[2][c];
+            const double no = pair.beta[0] * pair.x[0][c];
+            const double ng = pair.beta[1] * pair.x[1][c];
+            const double sum = no + ng;
+            if (!(remaining > 0.0) || !(sum > 0.0))
+                return false;
+            amount[0][c] = no * remaining / sum;
+            amount[1][c] = ng * remaining / sum;
+            logitsFromAmounts(z, amount, c, u);
+        }
+        return true;
+    }
+
+    // allocation.inc:503-582
+    __host__ __device__ bool allocationSeedFromSingle(double P, double T, const double *z, double *u)
+    {
+        const uint8_t active = chooseSinglePhase(P, T, z);
+        if (err.bad())
+            return false;
+        const Result single = singlePhaseResult(P, T, z, active);
+        if (err.bad())
+            return false;
+        const Stab st = stabilityTest(P, T, z, active, single.x);
+        if (err.bad())
+            return false;
+        if (!st.valid || st.stable)
+            return false;
+        double eps[3] = {0.0, 0.0, 0.0};
+        int missingCount = 0;
+        for (int p = 0; p < 3; ++p)
+            if (!(active & (1u << p)) && st.unstable[p])
+            {
+                eps[p] = 1.0e-4;
+                ++missingCount;
+            }
+        if (missingCount == 0)
+            return false;
+        for (int c = 0; c < NC; ++c)
+            for (int p = 0; p < 3; ++p)
+            {
+                if (!(eps[p] > 0.0))
+                    continue;
+                eps[p] = mind(eps[p], 0.08 * z[c] / maxd(st.incipient[p][c], 1.0e-30));
+            }
+        double amount[3][NC];
+        for (int p = 0; p < 3; ++p)
+            for (int c = 0; c < NC; ++c) amount[p][c] = 0.0;
+        const int activeSlot = firstActive(active);
+        for (int c = 0; c < NC; ++c)
+        {
+            if (!(z[c] > o.floor))
+            {
+                u[c] = 0.0;
+                u[NC + c] = 0.0;
+                continue;
+            }
+            double removed = 0.0;
+            for (int p = 0; p < 3; ++p)
+            {
+                if (!(eps[p] > 0.0))
+                    continue;
+                amount[p][c] = eps[p] * st.incipient[p][c];
+                removed += amount[p][c];
+            }
+            if (!(z[c] > removed))
+                return false;
+            amount[activeSlot][c] = z[c] - removed;
+            logitsFromAmounts(z, amount, c, u);
+        }
+        return true;
+    }
+
+    // allocation.inc:584-602
+    __host__ __device__ void thermodynamicAllocationSeed(double P, double T, double scale,
+                                                         double gasShift, double waterShift,
+                                                         double *u) const
+    {
+        double kg[NC], kw[NC];
+        wilsonK(P, T, kg);
+        initialWaterOilK(kw);
+        for (int c = 0; c < NC; ++c)
+        {
+            u[c] = clampd(scale * ::log(clampd(kg[c], 1.0e-30, 1.0e30)) + gasShift, -30.0, 30.0);
+            u[NC + c] = clampd(scale * ::log(clampd(kw[c], 1.0e-30, 1.0e30)) + waterShift, -30.0, 30.0);
+        }
+    }
+
+    // allocation.inc:609-700: 23 seeds (raw O/G pair, single phase, the three
+    // restricted pairs, 6 Wilson-scaled and 12 uniform logit biases), each
+    // preconditioned and refined; the lowest-Gibbs three-phase solution wins.
+    __host__ __device__ __attribute__((noinline)) Result flashThreeAllocationFallback(
+        double P, double T, const double *z)
+    {
+        Result best;
+        double bestG = __builtin_inf();
+        constexpr double thermoSeeds[6][3] = {{1.0, 0.0, 0.0},  {0.5, 0.0, 0.0},
+                                              {1.5, 0.0, 0.0},  {1.0, -1.5, -1.5},
+                                              {1.0, 1.5, -1.5}, {1.0, -1.5, 1.5}};
+        constexpr double generic[12][2] = {{0.0, 0.0},  {1.0, -2.0},  {2.0, -4.0}, {2.0, -2.0},
+                                           {1.0, -4.0}, {-1.0, -1.0}, {-2.0, -2.0}, {-2.0, 1.0},
+                                           {-1.0, 2.0}, {2.0, 1.0},   {1.0, 2.0},   {-2.0, 2.0}};
+        constexpr uint8_t pairs[3] = {kOilBit | kGasBit, kOilBit | kWaterBit, kGasBit | kWaterBit};
+        // Seeds that need flashes are built first, in the reference's order,
+        // and kept; the closed-form seeds are generated when their turn comes.
+        double built[5][kAllocDim];
+        bool builtValid[5];
+        builtValid[0] = allocationSeedFromRawOg(P, T, z, built[0]);
+        if (err.bad())
+            return best;
+        builtValid[1] = allocationSeedFromSingle(P, T, z, built[1]);
+        if (err.bad())
+            return best;
+        for (int k = 0; k < 3; ++k)
+        {
+            builtValid[2 + k] = allocationSeedFromPair(P, T, z, pairs[k], built[2 + k]);
+            if (err.bad())
+                return best;
+        }
+        Allocation state;
+        for (int seed = 0; seed < 23; ++seed)
+        {
+            double u[kAllocDim];
+            if (seed < 5)
+            {
+                if (!builtValid[seed])
+                    continue;
+                for (int i = 0; i < kAllocDim; ++i) u[i] = built[seed][i];
+            }
+            else if (seed < 11)
+            {
+                const double *spec = thermoSeeds[seed - 5];
+                thermodynamicAllocationSeed(P, T, spec[0], spec[1], spec[2], u);
+            }
+            else
+            {
+                for (int c = 0; c < NC; ++c)
+                {
+                    u[c] = generic[seed - 11][0];
+                    u[NC + c] = generic[seed - 11][1];
+                }
+            }
+            gibbsPreconditionAllocation(P, T, z, u, state);
+            if (err.bad())
+                return best;
+            if (!state.valid)
+                continue;
+            const bool refined = refineAllocationChemicalPotentials(P, T, z, u, state);
+            if (err.bad())
+                return best;
+            if (!refined)
+                continue;
+            if (!(state.beta[0] > o.betaTol) || !(state.beta[1] > o.betaTol) ||
+                !(state.beta[2] > o.betaTol))
+                continue;
+            if (!(state.gibbs < bestG))
+                continue;
+            Result cand;
+            cand.converged = true;
+            cand.presence = kAllBits;
+            for (int p = 0; p < 3; ++p)
+            {
+                cand.beta[p] = state.beta[p];
+                cand.Z[p] = state.Z[p];
+                for (int c = 0; c < NC; ++c) cand.x[p][c] = state.x[p][c];
+            }
+            deriveReferenceK(cand);
+            updateSaturationsFromMoles(P, T, cand);
+            if (err.bad())
+                return best;
+            if (!cand.converged)
+                continue;
+            best = cand;
+            bestG = state.gibbs;
+        }
+        return best;
+    }
+
     // ------------------------------------------------------------- flash
     // three_phase_flash.hpp:83-175: everything up to (and including) the SW
     // allocation stage.  `done` reports whether a result was returned early.
@@ -2370,9 +2931,27 @@ public:
 
         if (sw())
         {
-            // three_phase_flash.hpp:148-163: exact-mass allocation basin search
-            // (allocation.inc:609-700) is not provided by this build.
+            // three_phase_flash.hpp:153-170: exact-mass allocation basin search
             allocationFallbackReached = true;
+            Result allocated = flashThreeAllocationFallback(P, T, z);
+            if (err.bad())
+                return allocated;
+            if (allocated.converged)
+            {
+                canonicalizeFinal(P, T, allocated);
+                if (err.bad())
+                    return allocated;
+                if (allocated.converged && validatePhysical(P, T, z, allocated))
+                {
+                    const Result preferred = preferStableSwReducedOverSingle(P, T, z, allocated);
+                    if (err.bad())
+                        return preferred;
+                    if (preferred.converged)
+                        return preferred;
+                }
+                if (err.bad())
+                    return allocated;
+            }
         }
         done = false;
         return selected;

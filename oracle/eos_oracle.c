@@ -5,10 +5,10 @@
  * links it, and nothing in it is a fallback for a missing GPU.
  *
  * Restated functions (paths relative to the reference checkout):
- *   oracle_mixing        models/include/natural/thermo/cubic_eos.hpp:252-321
+ *   oracle_mixing        models/include/natural/thermo/cubic_eos.hpp:581-650
  *                        (+ thermo/soreide_whitson.hpp:26-44, cubic_eos.hpp:2389-2404)
- *   oracle_phase         cubic_eos.hpp:337-363 (phaseMixing), :366-474 (cubic roots,
- *                        liquid/vapour selection), :573-605 (fugacityAtRoot)
+ *   oracle_phase         cubic_eos.hpp:666-692 (phaseMixing), :695-803 (cubic roots,
+ *                        liquid/vapour selection), :902-934 (fugacityAtRoot)
  *   oracle_wilson_k      thermo/detail/three_phase_flash_core.inc:163-176
  *   oracle_rr2           thermo/detail/three_phase_flash_nonlinear.inc:129-187
  *   oracle_lbc_viscosity properties/compositional_properties.hpp:99-152,166-182
