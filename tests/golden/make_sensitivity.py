@@ -1,0 +1,121 @@
+"""Writes tests/golden/libm_sensitivity.json: which flash states of the SW test
+sets have results conditioned by the math library.
+
+For each state the host build of this library's flash (tests/golden/
+sw_sensitivity.cpp; on the host it reproduces the reference build to ~1e-15)
+is re-run under eight seeded 1-ulp perturbations of exp/log/pow/cos/acos.
+States whose values move by more than 1e-12 (relative, floor 1) — or whose
+label moves — are recorded with that band, and so is every state whose flash
+reaches the Soreide-Whitson exact-mass allocation recovery (a multi-start
+Gibbs minimisation, 23 seeds, argmin with strict comparisons, each run stopped
+at the 1e-9 fugacity tolerance): which seed wins, and so which of several
+equally converged answers and K-value records is returned, follows 1-ulp
+differences of the math library.  Those states' reference values are defined
+only up to the host libm; the GPU tests compare their status and labels
+exactly and report, without asserting, their value differences.
+
+    python tests/golden/make_sensitivity.py
+"""
+from __future__ import annotations
+
+import json
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+from nonenone_amd import fluids  # noqa: E402
+from nonenone_amd.workloads import batched_flash_states  # noqa: E402
+
+THRESHOLD = 1.0e-12
+SETS = {
+    # name: (fluid factory, source) — golden fixture file or the live seed-7 set
+    "flash_nc3_sw": (lambda: fluids.h2o_co2_nc10("SW"), "golden"),
+    "flash_nc6_sw": (lambda: fluids.six_component("SW"), "golden"),
+    "live_nc3_sw": (lambda: fluids.h2o_co2_nc10("SW"), "live"),
+    "live_nc6_sw": (lambda: fluids.six_component("SW"), "live"),
+}
+
+
+def build(tmp: Path) -> Path:
+    exe = tmp / "sw_sensitivity"
+    cmd = ["hipcc", "-std=c++17", "-O2", "-ffp-contract=off", "-x", "hip", "--offload-arch=gfx950",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           str(HERE / "sw_sensitivity.cpp"), "-o", str(exe)]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return exe
+
+
+def states(name, fluid, source):
+    if source == "golden":
+        g = np.load(HERE / f"{name}.npz")
+        return g["P"], g["T"], g["z"]
+    return batched_flash_states(256, fluid, seed=7, near_critical_fraction=0.1)
+
+
+def parse(stdout: str) -> dict:
+    flagged = {}
+    for line in stdout.split("\n"):
+        if not line.strip():
+            continue
+        idx, _, flips, band, alloc = line.split()
+        if int(flips) > 0 or float(band) > THRESHOLD or int(alloc):
+            flagged[idx] = {"label_flips": int(flips), "band": float(band),
+                            "allocation_recovery": int(alloc)}
+    return flagged
+
+
+def update_sets(exe: Path, tmp: Path) -> dict:
+    """The phase-update inputs of the SW grid test (60 x 20, seed 21, no well)."""
+    import assemblycmp
+    import refharness
+    from nonenone_amd._abi import FlashOptions
+
+    fluid = fluids.h2o_co2_nc10("SW")
+    ref = refharness.Reference(fluid, FlashOptions.defaults())
+    w, _, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 60, 20, 21, ref)
+    (tmp / "desc.bin").write_bytes(bytes(fluid))
+    with open(tmp / "cells.bin", "wb") as f:
+        np.array([len(cur), 0], np.int64).tofile(f)
+        np.ascontiguousarray(w["T"], np.float64).tofile(f)
+        np.ascontiguousarray(cur, np.float64).tofile(f)
+        np.ascontiguousarray(hist_ps, np.float64).tofile(f)
+    res = subprocess.run([str(exe), "update", str(tmp / "desc.bin"), str(tmp / "cells.bin")],
+                         check=True, capture_output=True, text=True)
+    flagged = parse(res.stdout)
+    print(f"update_sw_60x20_seed21: {len(flagged)} of {len(cur)} cells libm-conditioned", flush=True)
+    return {"update_sw_60x20_seed21": {"n": int(len(cur)), "conditioned": flagged}}
+
+
+def main() -> None:
+    out = {"threshold": THRESHOLD, "sets": {}}
+    with tempfile.TemporaryDirectory() as td:
+        tmp = Path(td)
+        exe = build(tmp)
+        out["sets"].update(update_sets(exe, tmp))
+        for name, (factory, source) in SETS.items():
+            fluid = factory()
+            P, T, z = states(name, fluid, source)
+            (tmp / "desc.bin").write_bytes(bytes(fluid))
+            with open(tmp / "states.bin", "wb") as f:
+                np.array([len(P)], np.int64).tofile(f)
+                np.ascontiguousarray(P, np.float64).tofile(f)
+                np.ascontiguousarray(T, np.float64).tofile(f)
+                np.ascontiguousarray(z, np.float64).tofile(f)
+            res = subprocess.run([str(exe), "flash", str(tmp / "desc.bin"), str(tmp / "states.bin")],
+                                 check=True, capture_output=True, text=True)
+            flagged = parse(res.stdout)
+            out["sets"][name] = {"n": int(len(P)), "conditioned": flagged}
+            print(f"{name}: {len(flagged)} of {len(P)} states libm-conditioned", flush=True)
+    (HERE / "libm_sensitivity.json").write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
+
+
+if __name__ == "__main__":
+    main()

