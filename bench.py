@@ -198,7 +198,7 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     lat = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
     kern = {}
-    for kind in ("phase_update", "cell_record", "assemble"):
+    for kind in ("phase_update", "phase_update_coop", "cell_record", "assemble"):
         ms, count = fl.kernel_time(kind)
         kern[kind] = dict(total_ms=ms, launches=count, avg_ms=ms / max(count, 1))
     fl.set_kernel_timing(False)

@@ -136,7 +136,9 @@ const char *nn_last_error(void);
 #define NN_KERNEL_PHASE_UPDATE 3
 #define NN_KERNEL_CELL_RECORD 4
 #define NN_KERNEL_ASSEMBLE 5
-#define NN_KERNEL_KINDS 6
+#define NN_KERNEL_FLASH_COOP 6        /* wavefront-per-state pass of nn_flash_*       */
+#define NN_KERNEL_PHASE_UPDATE_COOP 7 /* wavefront-per-cell pass of nn_phase_update  */
+#define NN_KERNEL_KINDS 8
 int nn_context_set_kernel_timing(nn_context *ctx, int32_t enable);
 int nn_context_kernel_time(nn_context *ctx, int32_t kind, double *total_ms,
                            int64_t *launches);

@@ -178,7 +178,7 @@ EXPORTED_SYMBOLS = (
 
 # Kernel kinds of nn_context_kernel_time.
 KERNEL_KINDS = {"eos": 0, "flash": 1, "stability": 2, "phase_update": 3, "cell_record": 4,
-                "assemble": 5}
+                "assemble": 5, "flash_coop": 6, "phase_update_coop": 7}
 
 
 def check(rc: int, what: str) -> None:

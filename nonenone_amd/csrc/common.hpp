@@ -34,7 +34,68 @@ enum : int
     kErrNone = 0,
     kErrThermo = NN_STATUS_THERMO_ERROR,
     kErrInvalid = NN_STATUS_INVALID_INPUT,
+    // Internal: the state exceeded the lane-per-state work budget and is handed
+    // to the wave-cooperative pass; never reported to callers.
+    kErrDefer = 64,
 };
+
+// Execution modes of the flash code.
+//   kModePlain: one work-item per state, everything serial in that work-item.
+//   kModeDefer: as plain, but a state whose Rachford-Rice recovery work passes
+//               kDeferBudget, or that enters the SW allocation recovery, stops
+//               with kErrDefer so the wave-cooperative pass can redo it.
+//   kModeCoop:  one wavefront per state; every lane runs the state's serial
+//               control flow on identical data, and the independent candidates
+//               of the recovery searches (RR3 starting guesses, allocation
+//               seeds) run on separate lanes.  Results are bit-identical to the
+//               serial order: the reductions keep the reference's first-wins rules.
+enum : int
+{
+    kModePlain = 0,
+    kModeDefer = 1,
+    kModeCoop = 2,
+};
+constexpr int kDeferBudget = 20000;  // RR3 residual evaluations per state
+
+// Wavefront helpers for kModeCoop (the host pass sees single-lane stand-ins).
+__host__ __device__ inline int waveLane()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __lane_id();
+#else
+    return 0;
+#endif
+}
+__host__ __device__ inline unsigned long long waveBallot(bool pred)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(pred);
+#else
+    return pred ? 1ull : 0ull;
+#endif
+}
+__host__ __device__ inline double waveShfl(double v, int src)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __shfl(v, src);
+#else
+    (void)src;
+    return v;
+#endif
+}
+__host__ __device__ inline int waveShfl(int v, int src)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __shfl(v, src);
+#else
+    (void)src;
+    return v;
+#endif
+}
+__host__ __device__ inline int firstSetLane(unsigned long long mask)
+{
+    return mask ? __builtin_ctzll(mask) : -1;
+}
 
 // Fluid parameters resident in HBM; kernels read them through uniform loads.
 struct FluidParams

@@ -145,7 +145,7 @@ __host__ __device__ bool solveDense(double (&a)[M][M], double (&rhs)[M], double 
     return true;
 }
 
-template <int NC>
+template <int NC, int Mode = kModePlain>
 class Flash
 {
 public:
@@ -163,6 +163,8 @@ public:
     // Set when a state reaches the Soreide-Whitson allocation basin search,
     // which this build does not provide (the state then reports not converged).
     bool allocationFallbackReached = false;
+    // RR3 residual evaluations so far (kModeDefer's budget; per lane in kModeCoop).
+    mutable int work = 0;
 
     __host__ __device__ Flash(const FluidParams &fp, const FlashOptions &op, Err &e)
         : f(fp), o(op), err(e), enablePermutation(true), enableContinuation(true)
@@ -478,6 +480,7 @@ public:
         for (int it = 0; it < o.maxRrIterations; ++it)
         {
             double fg, fw, jgg, jww, jgw;
+            ++work;
             const double norm = rr3Norm(z, kg, kw, bg, bw, &fg, &fw, &jgg, &jww, &jgw);
             if (!finite(norm))
                 return {false, 0.0, 0.0};
@@ -494,6 +497,7 @@ public:
             {
                 const double cg = bg + lambda * dbg;
                 const double cw = bw + lambda * dbw;
+                ++work;
                 const double cn = rr3Norm(z, kg, kw, cg, cw, nullptr, nullptr, nullptr, nullptr, nullptr);
                 if (finite(cn) && cn < norm)
                 {
@@ -510,22 +514,45 @@ public:
         return {false, 0.0, 0.0};
     }
 
-    // nonlinear.inc:95-127
+    // nonlinear.inc:95-127: the first of 17 starting guesses that converges.
     __host__ __device__ Rr3 solveRr3Robust(const double *z, const double *kg, const double *kw,
-                                           double pg, double pw) const
+                                           double pg, double pw)
     {
         const double guesses[17][2] = {
             {pg, pw}, {0.333, 0.333}, {0.50, 1.0e-4}, {1.0e-4, 0.50}, {0.80, 1.0e-4},
             {1.0e-4, 0.80}, {0.20, 0.20}, {0.60, 0.20}, {0.20, 0.60}, {0.05, 0.05},
             {0.90, 0.01}, {0.95, 1.0e-4}, {0.98, 1.0e-4}, {1.0e-4, 0.95}, {1.0e-4, 0.98},
             {0.01, 0.90}, {0.45, 0.45}};
-        for (int g = 0; g < 17; ++g)
+        if constexpr (Mode == kModeCoop)
         {
-            const Rr3 r = solveRr3(z, kg, kw, guesses[g][0], guesses[g][1]);
-            if (r.converged)
-                return r;
+            // Guess g on lane g; the lowest converged lane is the serial loop's answer.
+            const int lane = waveLane();
+            Rr3 r{false, 0.0, 0.0};
+            if (lane < 17)
+                r = solveRr3(z, kg, kw, guesses[lane][0], guesses[lane][1]);
+            const int first = firstSetLane(waveBallot(r.converged));
+            if (first < 0)
+                return {false, 0.0, 0.0};
+            return {true, waveShfl(r.bg, first), waveShfl(r.bw, first)};
         }
-        return {false, 0.0, 0.0};
+        else
+        {
+            if constexpr (Mode == kModeDefer)
+            {
+                if (work > kDeferBudget)
+                {
+                    err.raise(kErrDefer);
+                    return {false, 0.0, 0.0};
+                }
+            }
+            for (int g = 0; g < 17; ++g)
+            {
+                const Rr3 r = solveRr3(z, kg, kw, guesses[g][0], guesses[g][1]);
+                if (r.converged)
+                    return r;
+            }
+            return {false, 0.0, 0.0};
+        }
     }
 
     __host__ __device__ double rr2Eval(const double *z, const double *K, double beta, double *df) const
@@ -2766,6 +2793,11 @@ public:
     {
         Result best;
         double bestG = __builtin_inf();
+        if constexpr (Mode == kModeDefer)
+        {
+            err.raise(kErrDefer);
+            return best;
+        }
         constexpr double thermoSeeds[6][3] = {{1.0, 0.0, 0.0},  {0.5, 0.0, 0.0},
                                               {1.5, 0.0, 0.0},  {1.0, -1.5, -1.5},
                                               {1.0, 1.5, -1.5}, {1.0, -1.5, 1.5}};
@@ -2790,13 +2822,11 @@ public:
                 return best;
         }
         Allocation state;
-        for (int seed = 0; seed < 23; ++seed)
-        {
-            double u[kAllocDim];
+        auto seedVector = [&](int seed, double *u) {
             if (seed < 5)
             {
                 if (!builtValid[seed])
-                    continue;
+                    return false;
                 for (int i = 0; i < kAllocDim; ++i) u[i] = built[seed][i];
             }
             else if (seed < 11)
@@ -2812,6 +2842,70 @@ public:
                     u[NC + c] = generic[seed - 11][1];
                 }
             }
+            return true;
+        };
+        if constexpr (Mode == kModeCoop)
+        {
+            // Seed s is searched on lane s; the selection below then walks the
+            // seeds in order with the serial loop's rules (first error aborts,
+            // strict Gibbs improvement, invalid candidates skipped).
+            const int lane = waveLane();
+            bool ok = false;
+            int seedErr = kErrNone;
+            if (lane < 23)
+            {
+                double u[kAllocDim];
+                if (seedVector(lane, u))
+                {
+                    gibbsPreconditionAllocation(P, T, z, u, state);
+                    if (!err.bad() && state.valid)
+                    {
+                        const bool refined = refineAllocationChemicalPotentials(P, T, z, u, state);
+                        ok = !err.bad() && refined && state.beta[0] > o.betaTol &&
+                             state.beta[1] > o.betaTol && state.beta[2] > o.betaTol;
+                    }
+                }
+                seedErr = err.code;
+                err.code = kErrNone;
+            }
+            for (int seed = 0; seed < 23; ++seed)
+            {
+                const int e = waveShfl(seedErr, seed);
+                if (e != kErrNone)
+                {
+                    err.raise(e);
+                    return best;
+                }
+                if (!waveShfl(int(ok), seed))
+                    continue;
+                const double g = waveShfl(state.gibbs, seed);
+                if (!(g < bestG))
+                    continue;
+                Result cand;
+                cand.converged = true;
+                cand.presence = kAllBits;
+                for (int p = 0; p < 3; ++p)
+                {
+                    cand.beta[p] = waveShfl(state.beta[p], seed);
+                    cand.Z[p] = waveShfl(state.Z[p], seed);
+                    for (int c = 0; c < NC; ++c) cand.x[p][c] = waveShfl(state.x[p][c], seed);
+                }
+                deriveReferenceK(cand);
+                updateSaturationsFromMoles(P, T, cand);
+                if (err.bad())
+                    return best;
+                if (!cand.converged)
+                    continue;
+                best = cand;
+                bestG = g;
+            }
+            return best;
+        }
+        for (int seed = 0; seed < 23; ++seed)
+        {
+            double u[kAllocDim];
+            if (!seedVector(seed, u))
+                continue;
             gibbsPreconditionAllocation(P, T, z, u, state);
             if (err.bad())
                 return best;
@@ -2982,7 +3076,9 @@ public:
                 double zc[NC];
                 for (int c = 0; c < NC; ++c) zc[c] = zin[c];
                 bool d2 = false;
+                cand.work = work;
                 Result permuted = cand.flashMain(P, T, zc, d2);
+                work = cand.work;
                 allocationFallbackReached = allocationFallbackReached || cand.allocationFallbackReached;
                 if (err.bad())
                     return permuted;
@@ -3046,7 +3142,9 @@ public:
             double za[NC];
             for (int c = 0; c < NC; ++c) za[c] = z[c];
             bool d2 = false;
+            anchorSolver.work = work;
             Result anchor = anchorSolver.flashMain(anchorP, T, za, d2);
+            work = anchorSolver.work;
             allocationFallbackReached = allocationFallbackReached || anchorSolver.allocationFallbackReached;
             if (err.bad())
                 return anchor;
@@ -3086,8 +3184,8 @@ public:
 };
 
 // restricted.inc:8-61
-template <int NC>
-__host__ __device__ FlashResult<NC> Flash<NC>::flashRestrictedImpl(double P, double T, const double *zin,
+template <int NC, int Mode>
+__host__ __device__ FlashResult<NC> Flash<NC, Mode>::flashRestrictedImpl(double P, double T, const double *zin,
                                                                  uint8_t allowed, const double (*seed)[NC])
 {
     Result fail;

@@ -69,13 +69,15 @@ __device__ void storeFlash(const FlashResult<NC> &r, int64_t s, const nn_flash_o
     }
 }
 
-template <int NC>
-__device__ void flashOne(const FluidParams &f, const FlashOptions &o, int64_t s,
+// One state of a flash batch.  Returns false when a kModeDefer run handed the
+// state to the cooperative pass (nothing stored); in kModeCoop only lane 0 stores.
+template <int NC, int Mode>
+__device__ bool flashOne(const FluidParams &f, const FlashOptions &o, int64_t s,
                          const double *P, const double *T, const double *z,
                          const uint8_t *allowed, const double *seed, const nn_flash_out &out)
 {
     Err err;
-    Flash<NC> fl(f, o, err);
+    Flash<NC, Mode> fl(f, o, err);
     double zs[NC];
     for (int c = 0; c < NC; ++c) zs[c] = z[s * NC + c];
     FlashResult<NC> r;
@@ -114,6 +116,10 @@ __device__ void flashOne(const FluidParams &f, const FlashOptions &o, int64_t s,
             r = fl.flashRestricted(P[s], T[s], zs, a, sd);
         }
     }
+    if (err.code == kErrDefer)
+        return false;
+    if (Mode == kModeCoop && waveLane() != 0)
+        return true;
     storeFlash<NC>(r, s, out);
     int st = err.code;
     if (st == kErrNone)
@@ -121,6 +127,27 @@ __device__ void flashOne(const FluidParams &f, const FlashOptions &o, int64_t s,
     if (err.bad())
         out.converged[s] = 0;
     out.status[s] = st;
+    return true;
+}
+
+// Defers item s of a lane-per-item pass to the cooperative pass.
+__device__ inline void deferItem(WorkQueue *q, unsigned int *list, int64_t s)
+{
+    const unsigned int slot = atomicAdd(&q->deferred, 1u);
+    list[slot] = static_cast<unsigned int>(s);
+}
+
+// Next deferred item for this wavefront (uniform across its lanes), or -1.
+__device__ inline int64_t nextDeferred(WorkQueue *q, const unsigned int *list)
+{
+    unsigned int idx = 0;
+    if (waveLane() == 0)
+        idx = atomicAdd(&q->coopHead, 1u);
+    idx = static_cast<unsigned int>(waveShfl(static_cast<int>(idx), 0));
+    const unsigned int count = __atomic_load_n(&q->deferred, __ATOMIC_RELAXED);
+    if (idx >= count)
+        return -1;
+    return static_cast<int64_t>(list[idx]);
 }
 
 template <int NC>
@@ -128,15 +155,33 @@ __global__ void __launch_bounds__(kBlock) flashKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
     const uint8_t *__restrict__ allowed, const double *__restrict__ seed, nn_flash_out out,
-    unsigned int *__restrict__ counter)
+    WorkQueue *__restrict__ q, unsigned int *__restrict__ list)
 {
     // Persistent work queue: each work-item claims states until none are left.
     for (;;)
     {
-        const int64_t s = int64_t(atomicAdd(counter, 1u));
+        const int64_t s = int64_t(atomicAdd(&q->head, 1u));
         if (s >= n)
             break;
-        flashOne<NC>(*fp, *op, s, P, T, z, allowed, seed, out);
+        if (!flashOne<NC, kModeDefer>(*fp, *op, s, P, T, z, allowed, seed, out))
+            deferItem(q, list, s);
+    }
+}
+
+// One wavefront per deferred state (blocks of 64 = one wave).
+template <int NC>
+__global__ void __launch_bounds__(kBlock) flashCoopKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
+    const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
+    const uint8_t *__restrict__ allowed, const double *__restrict__ seed, nn_flash_out out,
+    WorkQueue *__restrict__ q, const unsigned int *__restrict__ list)
+{
+    for (;;)
+    {
+        const int64_t s = nextDeferred(q, list);
+        if (s < 0)
+            break;
+        flashOne<NC, kModeCoop>(*fp, *op, s, P, T, z, allowed, seed, out);
     }
 }
 
@@ -151,7 +196,7 @@ __global__ void __launch_bounds__(kBlock) stabilityKernel(
     if (s >= n)
         return;
     Err err;
-    Flash<NC> fl(*fp, *op, err);
+    Flash<NC, kModePlain> fl(*fp, *op, err);
     double zs[NC], xs[3][NC];
     for (int c = 0; c < NC; ++c) zs[c] = z[s * NC + c];
     for (int p = 0; p < 3; ++p)
@@ -175,14 +220,34 @@ template <int NC, bool Well>
 __global__ void __launch_bounds__(kBlock) phaseUpdateKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
-    int32_t *__restrict__ status, uint8_t *__restrict__ flags, unsigned int *__restrict__ counter)
+    int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
+    unsigned int *__restrict__ list)
 {
     for (;;)
     {
-        const int64_t s = int64_t(atomicAdd(counter, 1u));
+        const int64_t s = int64_t(atomicAdd(&q->head, 1u));
         if (s >= n)
             break;
-        phaseUpdateCell<NC, Well>(*fp, *op, T[s], primary, phaseState, s, status, flags);
+        if (!phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary, phaseState, s,
+                                                   status, flags))
+            deferItem(q, list, s);
+    }
+}
+
+template <int NC, bool Well>
+__global__ void __launch_bounds__(kBlock) phaseUpdateCoopKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
+    const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
+    int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
+    const unsigned int *__restrict__ list)
+{
+    for (;;)
+    {
+        const int64_t s = nextDeferred(q, list);
+        if (s < 0)
+            break;
+        phaseUpdateCell<NC, Well, kModeCoop>(*fp, *op, T[s], primary, phaseState, s, status,
+                                             flags);
     }
 }
 
@@ -395,12 +460,22 @@ hipError_t Launch<NC>::eos(unsigned grid, hipStream_t st, const FluidParams *fp,
 }
 
 template <int NC>
-hipError_t Launch<NC>::flash(unsigned grid, hipStream_t st, const FluidParams *fp,
-                             const FlashOptions *op, int64_t n, const double *P, const double *T,
-                             const double *z, const uint8_t *allowed, const double *seed,
-                             nn_flash_out out, unsigned int *counter)
+hipError_t Launch<NC>::flash(unsigned grid, unsigned coopGrid, hipStream_t st,
+                             const FluidParams *fp, const FlashOptions *op, int64_t n,
+                             const double *P, const double *T, const double *z,
+                             const uint8_t *allowed, const double *seed, nn_flash_out out,
+                             WorkQueue *q, unsigned int *list, hipEvent_t splitA,
+                             hipEvent_t splitB)
 {
-    flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, out, counter);
+    flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, out, q, list);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    if (splitA && (e = hipEventRecord(splitA, st)) != hipSuccess)
+        return e;
+    if (splitB && (e = hipEventRecord(splitB, st)) != hipSuccess)
+        return e;
+    flashCoopKernel<NC><<<coopGrid, kBlock, 0, st>>>(fp, op, P, T, z, allowed, seed, out, q, list);
     return hipGetLastError();
 }
 
@@ -418,17 +493,34 @@ hipError_t Launch<NC>::stability(unsigned grid, hipStream_t st, const FluidParam
 }
 
 template <int NC>
-hipError_t Launch<NC>::phaseUpdate(bool well, unsigned grid, hipStream_t st,
+hipError_t Launch<NC>::phaseUpdate(bool well, unsigned grid, unsigned coopGrid, hipStream_t st,
                                    const FluidParams *fp, const FlashOptions *op, int64_t n,
                                    const double *T, double *primary, double *phaseState,
-                                   int32_t *status, uint8_t *flags, unsigned int *counter)
+                                   int32_t *status, uint8_t *flags, WorkQueue *q,
+                                   unsigned int *list, hipEvent_t splitA,
+                             hipEvent_t splitB)
 {
     if (well)
         phaseUpdateKernel<NC, true><<<grid, kBlock, 0, st>>>(fp, op, n, T, primary, phaseState,
-                                                             status, flags, counter);
+                                                             status, flags, q, list);
     else
         phaseUpdateKernel<NC, false><<<grid, kBlock, 0, st>>>(fp, op, n, T, primary, phaseState,
-                                                              status, flags, counter);
+                                                              status, flags, q, list);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    if (splitA && (e = hipEventRecord(splitA, st)) != hipSuccess)
+        return e;
+    if (splitB && (e = hipEventRecord(splitB, st)) != hipSuccess)
+        return e;
+    if (well)
+        phaseUpdateCoopKernel<NC, true><<<coopGrid, kBlock, 0, st>>>(fp, op, T, primary,
+                                                                     phaseState, status, flags,
+                                                                     q, list);
+    else
+        phaseUpdateCoopKernel<NC, false><<<coopGrid, kBlock, 0, st>>>(fp, op, T, primary,
+                                                                      phaseState, status, flags,
+                                                                      q, list);
     return hipGetLastError();
 }
 

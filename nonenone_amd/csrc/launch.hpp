@@ -31,25 +31,40 @@ struct AssemblyView
     double *record;            // neighbour property records, field-major (SoA)
 };
 
+// Counters of a two-pass (lane-per-item, then wavefront-per-deferred-item) launch.
+struct WorkQueue
+{
+    unsigned int head;      // next item of the lane-per-item pass
+    unsigned int deferred;  // items handed to the cooperative pass (= list length)
+    unsigned int coopHead;  // next list entry of the cooperative pass
+    unsigned int pad;
+};
+
 template <int NC>
 struct Launch
 {
     static hipError_t eos(unsigned grid, hipStream_t st, const FluidParams *fp, int64_t n,
                           const double *P, const double *T, const double *x, const int32_t *role,
                           double *Z, double *lnphi, double *fug, int32_t *status);
-    static hipError_t flash(unsigned grid, hipStream_t st, const FluidParams *fp,
-                            const FlashOptions *op, int64_t n, const double *P, const double *T,
-                            const double *z, const uint8_t *allowed, const double *seed,
-                            nn_flash_out out, unsigned int *counter);
+    // Lane-per-state pass, then one wavefront per deferred state.  `splitA`
+    // and `splitB` (may be null) are recorded between the two launches.
+    static hipError_t flash(unsigned grid, unsigned coopGrid, hipStream_t st,
+                            const FluidParams *fp, const FlashOptions *op, int64_t n,
+                            const double *P, const double *T, const double *z,
+                            const uint8_t *allowed, const double *seed, nn_flash_out out,
+                            WorkQueue *q, unsigned int *list, hipEvent_t splitA,
+                            hipEvent_t splitB);
     static hipError_t stability(unsigned grid, hipStream_t st, const FluidParams *fp,
                                 const FlashOptions *op, int64_t n, const double *P,
                                 const double *T, const double *z, const uint8_t *active,
                                 const double *x, uint8_t *stable, uint8_t *valid, uint8_t *mask,
                                 double *trial, double *incipient, int32_t *status);
-    static hipError_t phaseUpdate(bool well, unsigned grid, hipStream_t st, const FluidParams *fp,
-                                  const FlashOptions *op, int64_t n, const double *T,
-                                  double *primary, double *phaseState, int32_t *status,
-                                  uint8_t *flags, unsigned int *counter);
+    static hipError_t phaseUpdate(bool well, unsigned grid, unsigned coopGrid, hipStream_t st,
+                                  const FluidParams *fp, const FlashOptions *op, int64_t n,
+                                  const double *T, double *primary, double *phaseState,
+                                  int32_t *status, uint8_t *flags, WorkQueue *q,
+                                  unsigned int *list, hipEvent_t splitA,
+                            hipEvent_t splitB);
     static hipError_t record(bool well, hipStream_t st, const FluidParams *fp, AssemblyView v,
                              const double *primary, const double *phaseState, double *accum,
                              int32_t *status);

@@ -28,9 +28,12 @@ struct nn_context
     FlashOptions hostOpts{};
     FluidParams *dFluid = nullptr;
     FlashOptions *dOpts = nullptr;
-    unsigned int *dCounter = nullptr;  // work-queue head for persistent kernels
+    WorkQueue *dQueue = nullptr;       // counters of the two-pass persistent launches
+    unsigned int *dList = nullptr;     // deferred item indices (capacity listCap)
+    int64_t listCap = 0;
     int numCUs = 256;
     int64_t maxBlocks = 0;  // NN_PERSISTENT_BLOCKS override of the persistent grid (0 = auto)
+    int64_t coopBlocks = 0; // NN_COOP_BLOCKS override of the cooperative grid (0 = auto)
     // Kernel timing: (start, end) event pairs per kernel kind while enabled.
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[NN_KERNEL_KINDS];
@@ -74,9 +77,15 @@ int dispatchNc(int nc, F &&f)
 }
 
 
-// Work-queue head reset by a kernel rather than hipMemsetAsync, so that the
-// launch sequence stays valid inside a captured HIP graph.
-__global__ void zeroCounterKernel(unsigned int *counter) { *counter = 0u; }
+// Work-queue counters reset by a kernel rather than hipMemsetAsync, so that
+// the launch sequence stays valid inside a captured HIP graph.
+__global__ void resetQueueKernel(WorkQueue *q)
+{
+    q->head = 0u;
+    q->deferred = 0u;
+    q->coopHead = 0u;
+    q->pad = 0u;
+}
 
 void releaseEvents(nn_context *ctx)
 {
@@ -114,12 +123,66 @@ struct LaunchTimer
             (void)hipEventDestroy(start);
         start = nullptr;
     }
+    // Events between the two passes of a two-pass launch (null when not
+    // timing): `splitA` ends the first interval, `splitB` starts the second.
+    hipEvent_t splitA = nullptr, splitB = nullptr;
+    void makeSplit()
+    {
+        if (!start)
+            return;
+        if (hipEventCreate(&splitA) != hipSuccess)
+            splitA = nullptr;
+        if (hipEventCreate(&splitB) != hipSuccess)
+            splitB = nullptr;
+    }
+    // Ends a two-pass launch: (start, splitA) under `kind`, (splitB, end) under `coopKind`.
+    void doneSplit(int coopKind)
+    {
+        if (!start || !splitA || !splitB)
+        {
+            done();
+            return;
+        }
+        hipEvent_t end = nullptr;
+        if (hipEventCreate(&end) == hipSuccess && hipEventRecord(end, st) == hipSuccess)
+        {
+            ctx->events[kind].emplace_back(start, splitA);
+            ctx->events[coopKind].emplace_back(splitB, end);
+            start = splitA = splitB = nullptr;
+        }
+    }
     ~LaunchTimer()
     {
         if (start)
             (void)hipEventDestroy(start);
+        if (splitA)
+            (void)hipEventDestroy(splitA);
+        if (splitB)
+            (void)hipEventDestroy(splitB);
     }
 };
+
+// Makes room for n deferred indices (synchronous allocation when it grows).
+int ensureList(nn_context *ctx, int64_t n)
+{
+    if (n <= ctx->listCap)
+        return NN_OK;
+    if (ctx->dList)
+    {
+        (void)hipFree(ctx->dList);
+        ctx->dList = nullptr;
+        ctx->listCap = 0;
+    }
+    if (hipMalloc(&ctx->dList, sizeof(unsigned int) * size_t(n)) != hipSuccess)
+        return fail(NN_E_OUT_OF_MEMORY, "device allocation of the deferred list failed");
+    ctx->listCap = n;
+    return NN_OK;
+}
+
+unsigned int coopGrid(const nn_context *ctx)
+{
+    return static_cast<unsigned int>(ctx->coopBlocks > 0 ? ctx->coopBlocks : int64_t(ctx->numCUs) * 4);
+}
 
 unsigned int persistentGrid(const nn_context *ctx, int64_t n)
 {
@@ -267,13 +330,15 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
         ctx->numCUs = prop.multiProcessorCount;
     if (const char *env = std::getenv("NN_PERSISTENT_BLOCKS"))
         ctx->maxBlocks = std::max(0L, std::atol(env));
+    if (const char *env = std::getenv("NN_COOP_BLOCKS"))
+        ctx->coopBlocks = std::max(0L, std::atol(env));
     nn_flash_options defaults;
     nn_flash_options_default(&defaults);
     ctx->host = toParams(*fluid);
     ctx->hostOpts = toOptions(opts ? *opts : defaults);
     if (hipMalloc(&ctx->dFluid, sizeof(FluidParams)) != hipSuccess ||
         hipMalloc(&ctx->dOpts, sizeof(FlashOptions)) != hipSuccess ||
-        hipMalloc(&ctx->dCounter, sizeof(unsigned int)) != hipSuccess)
+        hipMalloc(&ctx->dQueue, sizeof(WorkQueue)) != hipSuccess)
     {
         nn_context_destroy(ctx);
         return fail(NN_E_OUT_OF_MEMORY, "device allocation failed");
@@ -318,7 +383,8 @@ int nn_context_destroy(nn_context *ctx)
     releaseEvents(ctx);
     if (ctx->dFluid) (void)hipFree(ctx->dFluid);
     if (ctx->dOpts) (void)hipFree(ctx->dOpts);
-    if (ctx->dCounter) (void)hipFree(ctx->dCounter);
+    if (ctx->dQueue) (void)hipFree(ctx->dQueue);
+    if (ctx->dList) (void)hipFree(ctx->dList);
     delete ctx;
     return NN_OK;
 }
@@ -359,16 +425,20 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
         return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
     if (n > int64_t(0xFFFFFFF0u))
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
+    if (const int rc = ensureList(ctx, n); rc != NN_OK)
+        return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const nn_flash_out o = *out;
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        zeroCounterKernel<<<1, 1, 0, st>>>(ctx->dCounter);
+        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue);
         NN_HIP(hipGetLastError());
         LaunchTimer t(ctx, NN_KERNEL_FLASH, st);
-        NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), st, ctx->dFluid, ctx->dOpts, n, d_P, d_T,
-                                  d_z, d_allowed, d_seed, o, ctx->dCounter)));
-        t.done();
+        t.makeSplit();
+        NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), coopGrid(ctx), st, ctx->dFluid,
+                                  ctx->dOpts, n, d_P, d_T, d_z, d_allowed, d_seed, o, ctx->dQueue,
+                                  ctx->dList, t.splitA, t.splitB)));
+        t.doneSplit(NN_KERNEL_FLASH_COOP);
         return NN_OK;
     });
 }
@@ -427,16 +497,20 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
         return fail(NN_E_INVALID_ARGUMENT, "three-phase update needs a water component");
     if (n > int64_t(0xFFFFFFF0u))
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
+    if (const int rc = ensureList(ctx, n); rc != NN_OK)
+        return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        zeroCounterKernel<<<1, 1, 0, st>>>(ctx->dCounter);
+        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue);
         NN_HIP(hipGetLastError());
         LaunchTimer t(ctx, NN_KERNEL_PHASE_UPDATE, st);
-        NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, persistentGrid(ctx, n), st,
-                                        ctx->dFluid, ctx->dOpts, n, d_T, d_primary, d_phase_state,
-                                        d_status, d_flags, ctx->dCounter)));
-        t.done();
+        t.makeSplit();
+        NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, persistentGrid(ctx, n),
+                                        coopGrid(ctx), st, ctx->dFluid, ctx->dOpts, n, d_T,
+                                        d_primary, d_phase_state, d_status, d_flags, ctx->dQueue,
+                                        ctx->dList, t.splitA, t.splitB)));
+        t.doneSplit(NN_KERNEL_PHASE_UPDATE_COOP);
         return NN_OK;
     });
 }

@@ -65,7 +65,7 @@ enum : uint8_t
     kFlagStableReduced = 32,
 };
 
-template <int NC, bool Well>
+template <int NC, bool Well, int Mode = kModePlain>
 struct PhaseUpdater
 {
     using L = Layout<NC, Well>;
@@ -75,7 +75,7 @@ struct PhaseUpdater
     const FluidParams &f;
     const FlashOptions &o;
     Err &err;
-    Flash<NC> flash;
+    Flash<NC, Mode> flash;
     double T;
 
     __host__ __device__ PhaseUpdater(const FluidParams &fp, const FlashOptions &op, Err &e, double temp)
@@ -546,12 +546,15 @@ __host__ __device__ void encodePhaseState(const PhaseStateData<NC> &ps, double *
 }
 
 // One cell of nn_phase_update_batch (natural_petsc_runtime_state.inc:215-221).
-template <int NC, bool Well>
-__host__ __device__ void phaseUpdateCell(const FluidParams &f, const FlashOptions &o, double T,
+// Returns false when a kModeDefer run handed the cell to the cooperative pass
+// (nothing written); in kModeCoop only lane 0 writes.
+template <int NC, bool Well, int Mode = kModePlain>
+__host__ __device__ bool phaseUpdateCell(const FluidParams &f, const FlashOptions &o, double T,
                                          double *primary, double *phaseState, int64_t s,
                                          int32_t *status, uint8_t *flags)
 {
     using L = Layout<NC, Well>;
+    const bool writer = Mode != kModeCoop || waveLane() == 0;
     double prim[L::NP];
     double psv[L::NPS];
     for (int k = 0; k < L::NP; ++k) prim[k] = primary[s * L::NP + k];
@@ -560,28 +563,36 @@ __host__ __device__ void phaseUpdateCell(const FluidParams &f, const FlashOption
     if (!decodePhaseState<NC, Well>(psv, ps))
     {
         // decodePhasePresence throws std::runtime_error (phase_behavior.hpp:80-86).
-        status[s] = NN_STATUS_THERMO_ERROR;
-        if (flags) flags[s] = 0;
-        return;
+        if (writer)
+        {
+            status[s] = NN_STATUS_THERMO_ERROR;
+            if (flags) flags[s] = 0;
+        }
+        return true;
     }
     Err err;
-    PhaseUpdater<NC, Well> up(f, o, err, T);
+    PhaseUpdater<NC, Well, Mode> up(f, o, err, T);
     up.sanitize(prim);
     const int packed = up.update(prim, ps);
     const int st = packed >> 8;
     const uint8_t fl = uint8_t(packed & 0xff);
+    if (err.code == kErrDefer)
+        return false;
+    if (!writer)
+        return true;
     if (err.bad())
     {
         // The reference throws out of updatePhaseState; inputs stay untouched.
         status[s] = err.code;
         if (flags) flags[s] = fl;
-        return;
+        return true;
     }
     encodePhaseState<NC, Well>(ps, psv);
     for (int k = 0; k < L::NP; ++k) primary[s * L::NP + k] = prim[k];
     for (int k = 0; k < L::NPS; ++k) phaseState[s * L::NPS + k] = psv[k];
     status[s] = st;
     if (flags) flags[s] = fl;
+    return true;
 }
 
 } // namespace nn
