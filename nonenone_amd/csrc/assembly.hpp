@@ -21,6 +21,7 @@
 
 #include "common.hpp"
 #include "dual.hpp"
+#include "cpa.hpp"
 #include "eos.hpp"
 #include "phase_update.hpp"
 
@@ -161,28 +162,43 @@ __host__ __device__ void evaluateProperties(const FluidParams &f, const CellStat
     if (!(st.presence & kWaterBit))
         for (int c = 0; c < NC; ++c) comp[kWater][c] = S(c == f.water ? 1.0 : 0.0);
     const S T(Tcell);
+    const bool cpa = f.model == NN_MODEL_CPA;
     MixParams<NC, S> nonAq;
-    mixingParameters<NC, S>(f, st.p, T, kOil, nonAq);
+    CpaTemp<NC> cpaT;
+    if (cpa)
+        cpaTemperature<NC>(f, Tcell, cpaT);  // CPA: composition-dependent roots (:338-357)
+    else
+        mixingParameters<NC, S>(f, st.p, T, kOil, nonAq);
     const bool sw = f.model == NN_MODEL_SW;
     for (int q = 0; q < 3; ++q)
     {
         pr.sat[q] = st.sat[q];
-        MixParams<NC, S> aq;
-        const MixParams<NC, S> *mp = &nonAq;
-        if (q == kWater && sw)
-        {
-            mixingParameters<NC, S>(f, st.p, T, kWater, aq);
-            mp = &aq;
-        }
-        PhaseMix<NC, S> mix;
-        phaseMixing<NC, S>(comp[q], *mp, mix);
-        const S Z = selectRoot(f, mix.A, mix.B, q != kGas, err);
-        if (err.bad())
-            return;
+        S Z;
         S lnPhi[NC];
-        lnPhiAtRoot<NC, S>(f, *mp, mix, Z, lnPhi, err);
-        if (err.bad())
-            return;
+        if (cpa)
+        {
+            cpaPhase<NC, S>(f, cpaT, st.p, T, comp[q], q != kGas, Z, lnPhi, err);
+            if (err.bad())
+                return;
+        }
+        else
+        {
+            MixParams<NC, S> aq;
+            const MixParams<NC, S> *mp = &nonAq;
+            if (q == kWater && sw)
+            {
+                mixingParameters<NC, S>(f, st.p, T, kWater, aq);
+                mp = &aq;
+            }
+            PhaseMix<NC, S> mix;
+            phaseMixing<NC, S>(comp[q], *mp, mix);
+            Z = selectRoot(f, mix.A, mix.B, q != kGas, err);
+            if (err.bad())
+                return;
+            lnPhiAtRoot<NC, S>(f, *mp, mix, Z, lnPhi, err);
+            if (err.bad())
+                return;
+        }
         for (int c = 0; c < NC; ++c)
             pr.fug[q][c] = exp(lnPhi[c]) * st.p * comp[q][c];
         // eosFlowProperties: mass fractions, density, LBC viscosity

@@ -1,0 +1,492 @@
+// cpa.hpp — cubic-plus-association (CPA) phase evaluation, E6.
+//
+// Restates models/include/natural/thermo/cubic_eos.hpp (paths relative to the
+// reference checkout):
+//   temperature part of the mixing state   :1356-1414 (cache) / :1423-1539
+//   simplified radial distribution          :1541-1559
+//   association strength Delta_ij           :1561-1582
+//   site fractions: analytic symmetric      :1606-1623
+//                   single-donor Newton     :1624-1730
+//   pressure at density (SRK / PR term)     :2050-2078
+//   density root (bracket + bisection)      :2090-2178
+//   density root with two implicit-function
+//   corrections (first derivatives)         :2191-2248
+//   fugacity at density                     :2250-2307
+//   phase result / compressibility          :2309-2370, :864-882
+//
+// S is double (flash, phase update) or Dual<N> (AD properties).  Operations
+// keep the reference's operand types and order: quantities the reference holds
+// as AD scalars (temperature, composition, mixing sums) are S; per-temperature
+// constants, which carry no derivatives there, are double here — products and
+// sums with them round identically, and every division by a temperature
+// quantity keeps an S divisor.  The general multi-donor association solver
+// (:1731-2031) is not built; fluids that need it are rejected at context
+// creation.
+#pragma once
+
+#include "common.hpp"
+#include "dual.hpp"
+#include "eos.hpp"
+
+namespace nn
+{
+
+// Temperature-only part of the CPA mixing state (cpaTemperatureCache_).
+template <int NC>
+struct CpaTemp
+{
+    double T = -1.0;
+    double ai[NC], bi[NC], aij[NC][NC];
+    double tf[NC][NC];   // exp(eps_ij / (R T)) - 1, 0 when the pair does not associate
+    double vol[NC][NC];  // beta_ij
+};
+
+template <int NC, class S>
+struct CpaMix
+{
+    S Si[NC];
+    S a, b;
+};
+
+template <int NC, class S>
+struct CpaAssoc
+{
+    S donorX[NC], acceptorX[NC];
+    S g, gPrimeOverG, bonded;
+};
+
+// Which association solver the fluid needs (cubic_eos.hpp:1306-1353):
+// >= 0 a single symmetric associating component (analytic), or
+// -2 - k for a single donor-bearing component k, or -1 for none, or
+// kCpaGeneral when several components carry donor sites.
+constexpr int kCpaGeneral = -1000;
+
+__host__ __device__ inline int cpaScheme(const FluidParams &f)
+{
+    int assoc = -1;
+    bool symmetric = true;
+    for (int i = 0; i < f.nc; ++i)
+    {
+        const int d = f.cpaDonors[i], a = f.cpaAcceptors[i];
+        if (d == 0 && a == 0)
+            continue;
+        if (assoc >= 0 || d <= 0 || d != a)
+            symmetric = false;
+        assoc = i;
+    }
+    if (symmetric && assoc >= 0)
+    {
+        for (int i = 0; i < f.nc; ++i)
+            if (f.cpaDonors[i] > 0)
+                return i;
+    }
+    int donor = -1;
+    for (int i = 0; i < f.nc; ++i)
+    {
+        if (f.cpaDonors[i] <= 0)
+            continue;
+        if (donor >= 0)
+            return kCpaGeneral;
+        donor = i;
+    }
+    return donor >= 0 ? -2 - donor : -1;
+}
+
+// :1356-1414
+template <int NC>
+__host__ __device__ void cpaTemperature(const FluidParams &f, double T, CpaTemp<NC> &c)
+{
+    c.T = T;
+    double sqrtAi[NC];
+    for (int i = 0; i < NC; ++i)
+    {
+        const double tr = T / f.tc[i];
+        const double alphaRoot = 1.0 + f.cpaC1[i] * (1.0 - ::sqrt(tr));
+        c.ai[i] = f.cpaA0[i] * alphaRoot * alphaRoot;
+        c.bi[i] = f.cpaB[i];
+        sqrtAi[i] = ::sqrt(c.ai[i]);
+    }
+    for (int i = 0; i < NC; ++i)
+        for (int j = 0; j < NC; ++j)
+        {
+            c.aij[i][j] = sqrtAi[i] * sqrtAi[j] * (1.0 - f.kij[i][j]);
+            c.tf[i][j] = 0.0;
+            c.vol[i][j] = 0.0;
+            if (f.cpaDonors[i] <= 0 || f.cpaAcceptors[j] <= 0)
+                continue;
+            double eps = f.cpaCrossEps[i][j];
+            double beta = f.cpaCrossBeta[i][j];
+            if (!(eps > 0.0) || !(beta > 0.0))
+            {
+                if (f.cpaEps[i] > 0.0 && f.cpaEps[j] > 0.0 && f.cpaBeta[i] > 0.0 &&
+                    f.cpaBeta[j] > 0.0)
+                {
+                    eps = 0.5 * (f.cpaEps[i] + f.cpaEps[j]);
+                    beta = ::sqrt(f.cpaBeta[i] * f.cpaBeta[j]);
+                }
+                else
+                {
+                    eps = 0.0;
+                    beta = 0.0;
+                }
+            }
+            if (eps > 0.0 && beta > 0.0)
+            {
+                c.tf[i][j] = ::exp(eps / (kGasConstant * T)) - 1.0;
+                c.vol[i][j] = beta;
+            }
+        }
+}
+
+// Composition part of cpaMixing_ (:1520-1530).
+template <int NC, class S>
+__host__ __device__ void cpaMixing(const CpaTemp<NC> &c, const S *x, CpaMix<NC, S> &m)
+{
+    m.a = S(0.0);
+    m.b = S(0.0);
+    for (int i = 0; i < NC; ++i)
+    {
+        m.Si[i] = S(0.0);
+        for (int j = 0; j < NC; ++j) m.Si[i] += x[j] * c.aij[i][j];
+        m.a += x[i] * m.Si[i];
+        m.b += x[i] * c.bi[i];
+    }
+}
+
+// :1561-1582: Delta_ij = g (exp(eps/RT) - 1) b_ij beta_ij
+template <int NC, class S>
+__host__ __device__ S cpaDelta(const CpaTemp<NC> &c, int i, int j, const S &g)
+{
+    const double tf = c.tf[i][j];
+    const double beta = c.vol[i][j];
+    if (!(tf > 0.0) || !(beta > 0.0))
+        return S(0.0);
+    const double bij = 0.5 * (c.bi[i] + c.bi[j]);
+    return g * tf * bij * beta;
+}
+
+// :1584-2048 (analytic symmetric and single-donor solvers)
+template <int NC, class S>
+__host__ __device__ void cpaAssociation(const FluidParams &f, const CpaTemp<NC> &c, const S &rho,
+                                        const S *x, const CpaMix<NC, S> &m,
+                                        CpaAssoc<NC, S> &st, Err &err)
+{
+    for (int i = 0; i < NC; ++i)
+    {
+        st.donorX[i] = S(1.0);
+        st.acceptorX[i] = S(1.0);
+    }
+    const S eta = m.b * rho / 4.0;
+    // simplified radial distribution g = 1 / (1 - 1.9 eta) (:1541-1549)
+    const S den = S(1.0) - 1.9 * eta;
+    st.g = S(1.0) / den;
+    st.gPrimeOverG = 1.9 * st.g;
+
+    const int scheme = cpaScheme(f);
+    if (scheme >= 0)
+    {
+        const int k = scheme;
+        const int sites = f.cpaDonors[k];
+        const S delta = cpaDelta<NC, S>(c, k, k, st.g);
+        const S A = rho * x[k] * double(sites) * delta;
+        const S X = S(2.0) / (S(1.0) + sqrt(S(1.0) + S(4.0) * A));
+        st.donorX[k] = X;
+        st.acceptorX[k] = X;
+    }
+    else if (scheme <= -2)
+    {
+        const int donor = -2 - scheme;
+        S aC[NC], bC[NC];
+        S aSum = S(0.0);
+        for (int j = 0; j < NC; ++j)
+        {
+            aC[j] = S(0.0);
+            bC[j] = S(0.0);
+            if (f.cpaAcceptors[j] <= 0)
+                continue;
+            const S delta = cpaDelta<NC, S>(c, donor, j, st.g);
+            aC[j] = rho * x[j] * double(f.cpaAcceptors[j]) * delta;
+            bC[j] = rho * x[donor] * double(f.cpaDonors[donor]) * delta;
+            aSum += aC[j];
+        }
+        S donorX = S(1.0) / (S(1.0) + aSum);
+        double lower = 0.0, upper = 1.0;
+        bool converged = false;
+        const int maxIt = f.cpaMaxIter < 64 ? f.cpaMaxIter : 64;
+        for (int it = 0; it < maxIt; ++it)
+        {
+            S sum = S(0.0), slope = S(0.0);
+            for (int j = 0; j < NC; ++j)
+            {
+                if (f.cpaAcceptors[j] <= 0)
+                    continue;
+                const S d = S(1.0) + bC[j] * donorX;
+                sum += aC[j] / d;
+                slope -= aC[j] * bC[j] / (d * d);
+            }
+            const S residual = donorX * (S(1.0) + sum) - 1.0;
+            const S derivative = S(1.0) + sum + donorX * slope;
+            const double rv = sval(residual);
+            const double dv = sval(donorX);
+            if (rv > 0.0)
+                upper = dv;
+            else
+                lower = dv;
+            if (::fabs(rv) <= f.cpaTol)
+            {
+                converged = true;
+                break;
+            }
+            const double derv = sval(derivative);
+            S candidate = donorX;
+            if (finite(derv) && ::fabs(derv) > 1.0e-14)
+                candidate = donorX - residual / derivative;
+            const double cv = sval(candidate);
+            if (!finite(cv) || !(cv > lower && cv < upper))
+            {
+                const double mid = 0.5 * (lower + upper);
+                candidate = donorX + (mid - dv);
+            }
+            donorX = candidate;
+        }
+        if (!converged)
+        {
+            err.raise(kErrThermo);
+            return;
+        }
+        st.donorX[donor] = donorX;
+        for (int j = 0; j < NC; ++j)
+            if (f.cpaAcceptors[j] > 0)
+                st.acceptorX[j] = S(1.0) / (S(1.0) + bC[j] * donorX);
+    }
+    st.bonded = S(0.0);
+    for (int i = 0; i < NC; ++i)
+        st.bonded += x[i] * (double(f.cpaDonors[i]) * (S(1.0) - st.donorX[i]) +
+                             double(f.cpaAcceptors[i]) * (S(1.0) - st.acceptorX[i]));
+}
+
+// :2050-2078
+template <int NC, class S>
+__host__ __device__ S cpaPressure(const FluidParams &f, const CpaTemp<NC> &c, const S &rho,
+                                  const S &T, const S *x, const CpaMix<NC, S> &m, Err &err)
+{
+    if (!(sval(S(1.0) - m.b * rho) > 0.0))
+    {
+        err.raise(kErrThermo);
+        return S(0.0);
+    }
+    const S q = m.b * rho;
+    S attraction = S(1.0) + q;
+    if (f.cpaTerm == 1)
+    {
+        constexpr double sqrtTwo = 1.4142135623730950488;
+        attraction = (S(1.0) + (1.0 + sqrtTwo) * q) * (S(1.0) + (1.0 - sqrtTwo) * q);
+    }
+    const S cubic = kGasConstant * T * rho / (S(1.0) - q) - m.a * rho * rho / attraction;
+    CpaAssoc<NC, S> as;
+    cpaAssociation<NC, S>(f, c, rho, x, m, as, err);
+    if (err.bad())
+        return S(0.0);
+    const S factor = S(1.0) + as.gPrimeOverG * m.b * rho / 4.0;
+    const S association = -0.5 * kGasConstant * T * rho * factor * as.bonded;
+    return cubic + association;
+}
+
+// :2090-2178 (scalar density root: bracket, then up to 100 bisections)
+template <int NC>
+__host__ __device__ double cpaDensityRootValue(const FluidParams &f, const CpaTemp<NC> &c,
+                                               double P, double T, const double *x, bool liquid,
+                                               const CpaMix<NC, double> &m, Err &err)
+{
+    const double b = m.b;
+    if (!(b > 0.0))
+    {
+        err.raise(kErrThermo);
+        return 0.0;
+    }
+    const double rhoMax = 0.999999 / b;
+    auto residual = [&](double rho) {
+        return cpaPressure<NC, double>(f, c, rho, T, x, m, err) - P;
+    };
+    double lo = 0.0, hi = 0.0, flo = -P, fhi = 0.0;
+    if (!liquid)
+    {
+        hi = mind(rhoMax, maxd(1.0, 0.25 * P / (kGasConstant * T)));
+        fhi = residual(hi);
+        if (err.bad())
+            return 0.0;
+        while (fhi < 0.0 && hi < 0.999 * rhoMax)
+        {
+            lo = hi;
+            flo = fhi;
+            hi = mind(0.999999 * rhoMax, hi * 1.6 + 1.0);
+            fhi = residual(hi);
+            if (err.bad())
+                return 0.0;
+        }
+    }
+    else
+    {
+        hi = rhoMax;
+        fhi = residual(hi);
+        if (err.bad())
+            return 0.0;
+        lo = 0.82 * hi;
+        flo = residual(lo);
+        if (err.bad())
+            return 0.0;
+        for (int i = 0; i < 160 && flo > 0.0; ++i)
+        {
+            hi = lo;
+            fhi = flo;
+            lo *= 0.82;
+            flo = residual(lo);
+            if (err.bad())
+                return 0.0;
+        }
+    }
+    if (!(flo <= 0.0 && fhi >= 0.0))
+    {
+        err.raise(kErrThermo);
+        return 0.0;
+    }
+    for (int it = 0; it < 100; ++it)
+    {
+        const double mid = 0.5 * (lo + hi);
+        const double fm = residual(mid);
+        if (err.bad())
+            return 0.0;
+        if (fm > 0.0)
+        {
+            hi = mid;
+            fhi = fm;
+        }
+        else
+        {
+            lo = mid;
+            flo = fm;
+        }
+        if (::fabs(fm) <= 1.0e-10 * maxd(P, 1.0) || (hi - lo) <= 1.0e-12 * maxd(mid, 1.0))
+            break;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// :2191-2248: scalar root, then two implicit-function corrections
+// rho -= (p(rho) - P) / (dp/drho)_FD, which also carry first derivatives.
+template <int NC, class S>
+__host__ __device__ S cpaDensityRoot(const FluidParams &f, const CpaTemp<NC> &c, const S &P,
+                                     const S &T, const S *x, bool liquid, Err &err)
+{
+    double xs[NC];
+    for (int i = 0; i < NC; ++i) xs[i] = sval(x[i]);
+    const double pV = sval(P), tV = sval(T);
+    CpaMix<NC, double> sm;
+    cpaMixing<NC, double>(c, xs, sm);
+    const double root = cpaDensityRootValue<NC>(f, c, pV, tV, xs, liquid, sm, err);
+    S rho = P;
+    if (err.bad())
+        return rho;
+    rho *= 0.0;
+    rho += root;
+    CpaMix<NC, S> dm;
+    cpaMixing<NC, S>(c, x, dm);
+    for (int k = 0; k < 2; ++k)
+    {
+        const double rv = sval(rho);
+        const double h = maxd(1.0e-4, ::fabs(rv) * 2.0e-6);
+        const double lower = maxd(1.0e-12, rv - h);
+        const double upper = rv + h;
+        const double pLower = cpaPressure<NC, double>(f, c, lower, tV, xs, sm, err);
+        const double pUpper = cpaPressure<NC, double>(f, c, upper, tV, xs, sm, err);
+        if (err.bad())
+            return rho;
+        const double derivative = (pUpper - pLower) / (upper - lower);
+        if (!finite(derivative) || ::fabs(derivative) < 1.0e-12)
+        {
+            err.raise(kErrThermo);
+            return rho;
+        }
+        const S residual = cpaPressure<NC, S>(f, c, rho, T, x, dm, err) - P;
+        if (err.bad())
+            return rho;
+        rho -= residual / derivative;
+    }
+    return rho;
+}
+
+// :2250-2307: Z and ln(phi) at a density.
+template <int NC, class S>
+__host__ __device__ void cpaAtDensity(const FluidParams &f, const CpaTemp<NC> &c, const S &P,
+                                      const S &T, const S *x, const S &rho, S &Z, S *lnPhi,
+                                      Err &err)
+{
+    CpaMix<NC, S> m;
+    cpaMixing<NC, S>(c, x, m);
+    CpaAssoc<NC, S> as;
+    cpaAssociation<NC, S>(f, c, rho, x, m, as, err);
+    if (err.bad())
+        return;
+    const S q = m.b * rho;
+    if (!(sval(S(1.0) - q) > 0.0))
+    {
+        err.raise(kErrThermo);
+        return;
+    }
+    Z = P / (rho * kGasConstant * T);
+    S integral = log(S(1.0) + q);
+    S integralDerivative = S(1.0) / (S(1.0) + q);
+    if (f.cpaTerm == 1)
+    {
+        constexpr double sqrtTwo = 1.4142135623730950488;
+        constexpr double deltaOne = 1.0 + sqrtTwo;
+        constexpr double deltaTwo = 1.0 - sqrtTwo;
+        constexpr double deltaDifference = 2.0 * sqrtTwo;
+        const S first = S(1.0) + deltaOne * q;
+        const S second = S(1.0) + deltaTwo * q;
+        integral = log(first / second) / deltaDifference;
+        integralDerivative = S(1.0) / (first * second);
+    }
+    const S logZ = log(Z);
+    for (int i = 0; i < NC; ++i)
+    {
+        const double bi = c.bi[i];
+        S muCubic = -log(S(1.0) - q) + rho * bi / (S(1.0) - q);
+        muCubic -= ((2.0 * m.Si[i] / m.b - m.a * bi / (m.b * m.b)) * integral +
+                    (m.a / m.b) * rho * bi * integralDerivative) /
+                   (kGasConstant * T);
+        S muAssociation = S(0.0);
+        if (f.cpaDonors[i] > 0)
+            muAssociation += double(f.cpaDonors[i]) * log(as.donorX[i]);
+        if (f.cpaAcceptors[i] > 0)
+            muAssociation += double(f.cpaAcceptors[i]) * log(as.acceptorX[i]);
+        const S dLnGdNi = as.gPrimeOverG * bi * rho / 4.0;
+        muAssociation -= 0.5 * dLnGdNi * as.bonded;
+        lnPhi[i] = muCubic + muAssociation - logZ;
+    }
+}
+
+// :2309-2340 without the Gibbs-minimum root option: the selected root's Z and
+// ln(phi).  liquid = role is not gas.
+template <int NC, class S>
+__host__ __device__ void cpaPhase(const FluidParams &f, const CpaTemp<NC> &c, const S &P,
+                                  const S &T, const S *x, bool liquid, S &Z, S *lnPhi, Err &err)
+{
+    const S rho = cpaDensityRoot<NC, S>(f, c, P, T, x, liquid, err);
+    if (err.bad())
+        return;
+    cpaAtDensity<NC, S>(f, c, P, T, x, rho, Z, lnPhi, err);
+}
+
+// :864-882 compressibility(): Z of the selected root only.
+template <int NC>
+__host__ __device__ double cpaCompressibility(const FluidParams &f, const CpaTemp<NC> &c, double P,
+                                              double T, const double *x, bool liquid, Err &err)
+{
+    const double rho = cpaDensityRoot<NC, double>(f, c, P, T, x, liquid, err);
+    if (err.bad())
+        return 1.0;
+    return P / (rho * kGasConstant * T);
+}
+
+} // namespace nn

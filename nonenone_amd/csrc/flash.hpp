@@ -25,6 +25,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "cpa.hpp"
 #include "eos.hpp"
 
 // Optional control-flow trace (printf-style), compiled out unless defined.
@@ -93,6 +94,7 @@ struct ParamCache
     double P = 0.0, T = 0.0;
     MixParams<NC, double> nonAqueous;
     MixParams<NC, double> aqueous;
+    CpaTemp<NC> cpa;  // CPA: temperature part of the mixing state (cubic_eos.hpp:1356-1414)
 };
 
 // Gaussian elimination with partial pivoting (three_phase_flash_stability.inc:98-146).
@@ -163,6 +165,10 @@ public:
     // Set when a state reaches the Soreide-Whitson allocation basin search,
     // which this build does not provide (the state then reports not converged).
     bool allocationFallbackReached = false;
+    // Set when a lowest-Gibbs selection compared two candidates whose reduced
+    // Gibbs energies agree to 1e-12 (equivalent solutions): which one is kept
+    // then follows the last bits of the arithmetic.  Diagnostic only.
+    bool nearTie = false;
     // RR3 residual evaluations so far (kModeDefer's budget; per lane in kModeCoop).
     mutable int work = 0;
 
@@ -198,6 +204,16 @@ public:
     __host__ __device__ void thermoForRole(int role, double P, double T, const double *x,
                                            PhaseThermo<NC> &out)
     {
+        if (f.model == NN_MODEL_CPA)
+        {
+            // three_phase_flash_core.inc:108-112: the EOS's own role-aware CPA result
+            if (cache.cpa.T != T)
+                cpaTemperature<NC>(f, T, cache.cpa);
+            double lnPhi[NC];
+            cpaPhase<NC, double>(f, cache.cpa, P, T, x, role != kGas, out.Z, lnPhi, err);
+            for (int i = 0; i < NC; ++i) out.phi[i] = err.bad() ? 1.0 : ::exp(lnPhi[i]);
+            return;
+        }
         ensureCache(P, T);
         const MixParams<NC, double> &mp =
             (role == kWater && sw()) ? cache.aqueous : cache.nonAqueous;
@@ -712,6 +728,7 @@ public:
                 K[c] = clampd(K[c], 1.0e-100, 1.0e100);
             }
             ++total;
+            NN_TRACE("two ssi pair=%d%d it=%d res=%.17g beta=%.17g\n", first, second, total, maxResidual, betaSecond);
             result.iterations = total;
             result.beta[0] = result.beta[1] = result.beta[2] = 0.0;
             result.beta[first] = betaFirst;
@@ -754,6 +771,7 @@ public:
             result = base.result;
             result.iterations = ++total;
             betaSecond = result.beta[second];
+            NN_TRACE("two newton pair=%d%d it=%d norm=%.17g\n", first, second, total, base.norm);
             if (base.norm < o.fugTol)
             {
                 result.converged = true;
@@ -2229,6 +2247,7 @@ public:
             if (!st.valid || !st.stable)
                 continue;
             const double g = dimensionlessGibbs(P, T, cand);
+            noteGibbsTie(g, bestG);
             if (finite(g) && g < bestG)
             {
                 bestG = g;
@@ -2312,6 +2331,12 @@ public:
         n.presence = presenceFromFractions(n.beta);
         deriveReferenceK(n);
         return n;
+    }
+
+    __host__ __device__ void noteGibbsTie(double g, double bestG)
+    {
+        if (finite(g) && finite(bestG) && ::fabs(g - bestG) <= 1.0e-12 * maxd(1.0, ::fabs(bestG)))
+            nearTie = true;
     }
 
     // ------------------------------------------- SW exact-mass allocation
@@ -2879,6 +2904,7 @@ public:
                 if (!waveShfl(int(ok), seed))
                     continue;
                 const double g = waveShfl(state.gibbs, seed);
+                noteGibbsTie(g, bestG);
                 if (!(g < bestG))
                     continue;
                 Result cand;
@@ -2919,6 +2945,7 @@ public:
             if (!(state.beta[0] > o.betaTol) || !(state.beta[1] > o.betaTol) ||
                 !(state.beta[2] > o.betaTol))
                 continue;
+            noteGibbsTie(state.gibbs, bestG);
             if (!(state.gibbs < bestG))
                 continue;
             Result cand;
@@ -3080,6 +3107,7 @@ public:
                 Result permuted = cand.flashMain(P, T, zc, d2);
                 work = cand.work;
                 allocationFallbackReached = allocationFallbackReached || cand.allocationFallbackReached;
+                nearTie = nearTie || cand.nearTie;
                 if (err.bad())
                     return permuted;
                 if (!permuted.converged)
@@ -3087,6 +3115,7 @@ public:
                 const double g = cand.dimensionlessGibbs(P, T, permuted);
                 if (err.bad())
                     return permuted;
+                noteGibbsTie(g, bestG);
                 if (finite(g) && g < bestG)
                 {
                     bestG = g;
@@ -3146,6 +3175,7 @@ public:
             Result anchor = anchorSolver.flashMain(anchorP, T, za, d2);
             work = anchorSolver.work;
             allocationFallbackReached = allocationFallbackReached || anchorSolver.allocationFallbackReached;
+            nearTie = nearTie || anchorSolver.nearTie;
             if (err.bad())
                 return anchor;
             if (!anchor.converged)

@@ -28,14 +28,24 @@ __global__ void __launch_bounds__(kBlock) eosPhaseKernel(
     double xs[NC];
     for (int c = 0; c < NC; ++c) xs[c] = x[s * NC + c];
     const int r = role[s];
-    MixParams<NC, double> mp;
-    mixingParameters<NC, double>(f, P[s], T[s], r, mp);
-    PhaseMix<NC, double> mix;
-    phaseMixing<NC, double>(xs, mp, mix);
-    const double z = selectRoot(f, mix.A, mix.B, r != kGas, err);
+    double z = 1.0;
     double lp[NC];
-    if (!err.bad())
-        lnPhiAtRoot<NC, double>(f, mp, mix, z, lp, err);
+    if (f.model == NN_MODEL_CPA)
+    {
+        CpaTemp<NC> ct;
+        cpaTemperature<NC>(f, T[s], ct);
+        cpaPhase<NC, double>(f, ct, P[s], T[s], xs, r != kGas, z, lp, err);
+    }
+    else
+    {
+        MixParams<NC, double> mp;
+        mixingParameters<NC, double>(f, P[s], T[s], r, mp);
+        PhaseMix<NC, double> mix;
+        phaseMixing<NC, double>(xs, mp, mix);
+        z = selectRoot(f, mix.A, mix.B, r != kGas, err);
+        if (!err.bad())
+            lnPhiAtRoot<NC, double>(f, mp, mix, z, lp, err);
+    }
     Z[s] = z;
     for (int c = 0; c < NC; ++c)
     {

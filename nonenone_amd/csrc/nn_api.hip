@@ -17,6 +17,7 @@
 
 #include "../../include/nonenone_amd.h"
 #include "common.hpp"
+#include "cpa.hpp"
 #include "launch.hpp"
 
 using namespace nn;
@@ -310,7 +311,17 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
     if (!(fluid->omega_a > 0.0) || !(fluid->omega_b > 0.0) || fluid->m1 == fluid->m2)
         return fail(NN_E_INVALID_ARGUMENT, "invalid cubic EOS constants");
     if (fluid->thermo_model == NN_MODEL_CPA)
-        return fail(NN_E_UNSUPPORTED, "CPA is not provided by this build");
+    {
+        const FluidParams probe = toParams(*fluid);
+        if (cpaScheme(probe) == kCpaGeneral)
+            return fail(NN_E_UNSUPPORTED,
+                        "CPA with several donor-bearing components (general site solver) is not built");
+        if (fluid->cpa_physical_term != 0 && fluid->cpa_physical_term != 1)
+            return fail(NN_E_INVALID_ARGUMENT, "cpa_physical_term must be 0 (SRK) or 1 (PR)");
+        for (int i = 0; i < fluid->nc; ++i)
+            if (!(fluid->cpa_b[i] > 0.0) || !(fluid->cpa_a0[i] > 0.0))
+                return fail(NN_E_INVALID_ARGUMENT, "CPA a0 and b must be positive");
+    }
     if (fluid->thermo_model == NN_MODEL_SW &&
         (fluid->water_component < 0 || fluid->water_component >= fluid->nc))
         return fail(NN_E_INVALID_ARGUMENT, "SW water-component index is out of range");

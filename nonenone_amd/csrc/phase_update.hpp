@@ -259,18 +259,36 @@ struct PhaseUpdater
         double moles[3] = {0.0, 0.0, 0.0};
         const double P = prim[L::pressure];
         double total = 0.0;
+        const bool cpa = f.model == NN_MODEL_CPA;
         MixParams<NC, double> nonAq, aq;
-        mixingParameters<NC, double>(f, P, T, kOil, nonAq);
-        if (sw())
-            mixingParameters<NC, double>(f, P, T, kWater, aq);
+        CpaTemp<NC> cpaT;
+        if (cpa)
+        {
+            cpaTemperature<NC>(f, T, cpaT);
+        }
+        else
+        {
+            mixingParameters<NC, double>(f, P, T, kOil, nonAq);
+            if (sw())
+                mixingParameters<NC, double>(f, P, T, kWater, aq);
+        }
         for (int p = 0; p < 3; ++p)
         {
             if (!(ps.presence & (1u << p)))
                 continue;
-            const MixParams<NC, double> &mp = (p == kWater && sw()) ? aq : nonAq;
-            PhaseMix<NC, double> mix;
-            phaseMixing<NC, double>(comp[p], mp, mix);
-            const double Z = selectRoot(f, mix.A, mix.B, p != kGas, err);
+            double Z;
+            if (cpa)
+            {
+                // :188-200: eos.compressibility (CPA density root)
+                Z = cpaCompressibility<NC>(f, cpaT, P, T, comp[p], p != kGas, err);
+            }
+            else
+            {
+                const MixParams<NC, double> &mp = (p == kWater && sw()) ? aq : nonAq;
+                PhaseMix<NC, double> mix;
+                phaseMixing<NC, double>(comp[p], mp, mix);
+                Z = selectRoot(f, mix.A, mix.B, p != kGas, err);
+            }
             if (err.bad())
                 return;
             ps.Zf[p] = Z;

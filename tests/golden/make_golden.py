@@ -5,7 +5,7 @@ Run in a checkout that has /root/reference (oracle/_ref/libnn_ref.so built by
 generator, 10% near-critical states) and the reference's own outputs of
 CubicThreePhaseFlash::flash for them:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--all]
 """
 from __future__ import annotations
 
@@ -29,6 +29,7 @@ CASES = {
     "flash_nc3_sw": (lambda: fluids.h2o_co2_nc10("SW"), 102),
     "flash_nc6_pr": (lambda: fluids.six_component("PR"), 103),
     "flash_nc6_sw": (lambda: fluids.six_component("SW"), 104),
+    "flash_nc3_cpa": (fluids.cpa_h2o_co2_nc10, 105),
 }
 N = 192
 
@@ -63,10 +64,15 @@ def assembly_fixture():
 
 
 def main():
+    """Writes the fixtures that are missing (all of them with --all)."""
     if not refharness.available():
         raise SystemExit("oracle/_ref/libnn_ref.so missing: run `make -C oracle` first")
-    assembly_fixture()
+    everything = "--all" in sys.argv
+    if everything or not (HERE / "assembly_nc3_pr_12x8.npz").exists():
+        assembly_fixture()
     for name, (make, seed) in CASES.items():
+        if not everything and (HERE / f"{name}.npz").exists():
+            continue
         fluid = make()
         ref = refharness.Reference(fluid, FlashOptions.defaults())
         P, T, z = batched_flash_states(N, fluid, seed=seed, near_critical_fraction=0.1)

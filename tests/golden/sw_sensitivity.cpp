@@ -1,4 +1,5 @@
 // Libm-conditioning probe for flash fixtures (test tooling, host only).
+// (SW and CPA fluids; the file name is historical to its first use.)
 //
 // Runs this library's flash (nonenone_amd/csrc/flash.hpp, compiled for the
 // host, where it reproduces the reference build to ~1e-15) once with the host
@@ -8,15 +9,18 @@
 // are both within 1 ulp (glibc on the host, the device library on the GPU)
 // can legitimately give values that differ there by more than the parity
 // tolerance.  Output, one line per state:
-//     <index> <presence> <label_flips> <worst_relative_change> <allocation_recovery>
-// where the last field is 1 when the unperturbed flash reached the
-// Soreide-Whitson exact-mass allocation recovery (allocation.inc:609-700).
+//     <index> <presence> <label_flips> <worst_relative_change> <allocation_recovery> <near_tie>
+// where allocation_recovery is 1 when the unperturbed flash reached the
+// Soreide-Whitson exact-mass allocation recovery (allocation.inc:609-700) and
+// near_tie is 1 when one of its lowest-Gibbs selections compared two
+// candidates whose reduced Gibbs energies agree to 1e-12 (equivalent
+// solutions, chosen by the last bits of the arithmetic).
 //
 // The same for the hysteretic phase update (phase_update.hpp), with the
 // primary vector compared instead of the flash record.
 //
-// Usage: sw_sensitivity flash <fluid_desc.bin> <states.bin>
-//        sw_sensitivity update <fluid_desc.bin> <cells.bin>
+// Usage: sw_sensitivity flash <fluid_desc.bin> <states.bin> [salts]
+//        sw_sensitivity update <fluid_desc.bin> <cells.bin> [salts]
 //   fluid_desc.bin: raw nn_fluid_desc; states.bin: int64 n, then P[n], T[n], z[n][nc];
 //   cells.bin: int64 n, int64 has_well, then T[n], primary[n][NP], phase_state[n][NPS].
 #include <hip/hip_runtime.h>
@@ -27,6 +31,7 @@
 
 static uint64_t g_calls = 0, g_salt = 0;
 static int g_mode = 0;
+static int g_salts = 8;  // perturbed re-runs per state
 static inline double bump(double r)
 {
     if (!g_mode)
@@ -56,6 +61,7 @@ static double noisyAcos(double x) { return bump(std::acos(x)); }
 #undef acos
 
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 using namespace nn;
@@ -80,10 +86,25 @@ static FluidParams paramsFrom(const nn_fluid_desc &d)
         p.mw[i] = d.molar_mass[i];
         p.swAqKij[i] = d.sw_aqueous_kij[i];
         p.swAqSet[i] = d.sw_aqueous_kij_set[i];
+        p.cpaA0[i] = d.cpa_a0[i];
+        p.cpaB[i] = d.cpa_b[i];
+        p.cpaC1[i] = d.cpa_c1[i];
+        p.cpaEps[i] = d.cpa_epsilon[i];
+        p.cpaBeta[i] = d.cpa_beta[i];
+        p.cpaDonors[i] = d.cpa_donors[i];
+        p.cpaAcceptors[i] = d.cpa_acceptors[i];
         for (int j = 0; j < NN_MAX_NC; ++j)
+        {
             p.kij[i][j] = d.kij[i * NN_MAX_NC + j];
+            p.cpaCrossEps[i][j] = d.cpa_cross_epsilon[i * NN_MAX_NC + j];
+            p.cpaCrossBeta[i][j] = d.cpa_cross_beta[i * NN_MAX_NC + j];
+        }
     }
     p.swSalinity = d.sw_salinity;
+    p.cpaTerm = d.cpa_physical_term;
+    p.cpaMaxIter = d.cpa_max_iterations;
+    p.cpaTol = d.cpa_tolerance;
+    p.cpaDamping = d.cpa_damping;
     return p;
 }
 
@@ -123,7 +144,7 @@ static int run(const FluidParams &fp, const FlashOptions &op, int64_t n, const d
         const FlashResult<NC> r0 = f0.flash(P[s], T[s], z + s * NC);
         int flips = 0;
         double worst = 0.0;
-        for (int salt = 1; salt <= 8; ++salt)
+        for (int salt = 1; salt <= g_salts; ++salt)
         {
             g_mode = 1;
             g_salt = salt;
@@ -138,8 +159,8 @@ static int run(const FluidParams &fp, const FlashOptions &op, int64_t n, const d
             }
             worst = std::fmax(worst, change(r0, r1));
         }
-        std::printf("%lld %d %d %.6e %d\n", (long long)s, (int)r0.presence, flips, worst,
-                    int(f0.allocationFallbackReached));
+        std::printf("%lld %d %d %.6e %d %d\n", (long long)s, (int)r0.presence, flips, worst,
+                    int(f0.allocationFallbackReached), int(f0.nearTie));
     }
     return 0;
 }
@@ -156,9 +177,9 @@ static int runUpdate(const FluidParams &fp, const FlashOptions &op, int64_t n, F
     for (int64_t s = 0; s < n; ++s)
     {
         double p0[L::NP], p1[L::NP];
-        int alloc = 0, flips = 0, label0 = -1;
+        int alloc = 0, tie = 0, flips = 0, label0 = -1;
         double worst = 0.0;
-        for (int salt = 0; salt <= 8; ++salt)
+        for (int salt = 0; salt <= g_salts; ++salt)
         {
             g_mode = salt > 0;
             g_salt = salt;
@@ -177,6 +198,7 @@ static int runUpdate(const FluidParams &fp, const FlashOptions &op, int64_t n, F
             {
                 label0 = label;
                 alloc = int(up.flash.allocationFallbackReached);
+                tie = int(up.flash.nearTie);
                 continue;
             }
             if (label != label0)
@@ -192,15 +214,17 @@ static int runUpdate(const FluidParams &fp, const FlashOptions &op, int64_t n, F
                 worst = std::fmax(worst, std::fabs(p0[k] - p1[k]) / sc);
             }
         }
-        std::printf("%lld %d %d %.6e %d\n", (long long)s, label0, flips, worst, alloc);
+        std::printf("%lld %d %d %.6e %d %d\n", (long long)s, label0, flips, worst, alloc, tie);
     }
     return 0;
 }
 
 int main(int argc, char **argv)
 {
-    if (argc != 4)
+    if (argc != 4 && argc != 5)
         return 2;
+    if (argc == 5)
+        g_salts = std::atoi(argv[4]);
     const bool update = std::strcmp(argv[1], "update") == 0;
     nn_fluid_desc d;
     FILE *fd = std::fopen(argv[2], "rb");

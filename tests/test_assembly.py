@@ -93,14 +93,15 @@ def test_gpu_assembly_matches_fixture():
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
-@pytest.mark.parametrize("model,has_well", [("PR", False), ("SW", False), ("PR", True)])
+@pytest.mark.parametrize("model,has_well", [("PR", False), ("SW", False), ("PR", True),
+                                            ("CPA", False), ("CPA", True)])
 def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     import torch
     from nonenone_amd.assembly import NaturalAssembly
 
     from nonenone_amd.flash import ThreePhaseFlash
 
-    fluid = fluids.h2o_co2_nc10(model)
+    fluid = fluids.cpa_h2o_co2_nc10() if model == "CPA" else fluids.h2o_co2_nc10(model)
     ref = refharness.Reference(fluid, FlashOptions.defaults())
     w, hist_p, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 60, 20, 21, ref,
                                                                 has_well=has_well)

@@ -29,6 +29,7 @@ CASES = {
     "flash_nc3_sw": lambda: fluids.h2o_co2_nc10("SW"),
     "flash_nc6_pr": lambda: fluids.six_component("PR"),
     "flash_nc6_sw": lambda: fluids.six_component("SW"),
+    "flash_nc3_cpa": fluids.cpa_h2o_co2_nc10,
 }
 
 
@@ -189,3 +190,92 @@ def test_gpu_stability_matches_golden_single_phase():
     assert np.all(r["status"].cpu().numpy() == 0)
     assert np.all(r["valid"].cpu().numpy() == 1)
     assert np.all(r["stable"].cpu().numpy() == 1)
+
+
+# ------------------------------------------------------------------ CPA (E6)
+def _cpa_states(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    P = rng.uniform(1.0e6, 40.0e6, n)
+    T = rng.uniform(300.0, 500.0, n)
+    x = rng.dirichlet(np.ones(3), n)
+    role = rng.integers(0, 3, n).astype(np.int32)
+    return P, T, x, role
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+def test_gpu_cpa_eos_matches_reference_build():
+    """E6: role-aware CPA phase results (4C water + CO2 B2 cross-association,
+    single-donor site solver) against the reference build."""
+    fluid = fluids.cpa_h2o_co2_nc10()
+    P, T, x, role = _cpa_states(512, 31)
+    rZ, rlnphi, rfug, rst = refharness.Reference(fluid, FlashOptions.defaults()).eos_phase(P, T, x, role)
+    Z, lnphi, fug, st = (t.cpu().numpy() for t in _flash_gpu(fluid).eos_phase(P, T, x, role))
+    assert np.array_equal(st, rst)
+    ok = rst == 0
+    dz = np.abs(Z[ok] - rZ[ok]) / np.abs(rZ[ok])
+    df = np.abs(fug[ok] - rfug[ok]) / np.maximum(np.abs(rfug[ok]), 1e-300)
+    print(f"CPA EOS: {int(ok.sum())} states, max rel Z {dz.max():.2e}, fugacity {df.max():.2e}")
+    assert dz.max() <= 1e-10 and df.max() <= 1e-10
+
+
+def _cpa_without_association():
+    """H2O/CO2/nC10 PR-CPA with every association site removed and the cubic
+    parameters of PR78: a0 = 0.45724 R^2 Tc^2/Pc, b = 0.0778 R Tc/Pc, c1 = m(omega)."""
+    d = fluids.cpa_h2o_co2_nc10()
+    R = 8.31446261815324
+    for i in range(3):
+        tc, pc, om = d.tc[i], d.pc[i], d.acentric[i]
+        d.cpa_a0[i] = 0.45724 * R * R * tc * tc / pc
+        d.cpa_b[i] = 0.07780 * R * tc / pc
+        d.cpa_c1[i] = (0.379642 + 1.48503 * om - 0.164423 * om * om + 0.016666 * om ** 3
+                       if om > 0.49 else 0.37464 + 1.54226 * om - 0.26992 * om * om)
+        d.cpa_donors[i] = d.cpa_acceptors[i] = 0
+        d.cpa_epsilon[i] = d.cpa_beta[i] = 0.0
+    for k in range(NN_MAX_NC * NN_MAX_NC):
+        d.cpa_cross_epsilon[k] = d.cpa_cross_beta[k] = 0.0
+    return d
+
+
+@pytest.mark.gpu
+def test_gpu_cpa_dry_limit_recovers_pr():
+    """Non-associating PR-CPA is the PR78 cubic (thermo_empirical_parameters_test.cpp:698-739,
+    tolerances 2e-8 on Z and 3e-8 on phi), checked against the C oracle's PR."""
+    import oracle_c
+    cpa = _cpa_without_association()
+    pr = fluids.make_desc(["H2O", "CO2", "nC10"],
+                          [[0.0, 0.10566, 0.0], [0.10566, 0.0, 0.1141], [0.0, 0.1141, 0.0]],
+                          model="PR", eos_type=5)
+    P, T, x, role = _cpa_states(256, 32)
+    role = np.where(role == 2, 0, role).astype(np.int32)   # liquid or vapour root
+    Z, lnphi, _, st = (t.cpu().numpy() for t in _flash_gpu(cpa).eos_phase(P, T, x, role))
+    oZ, olnphi, ost = oracle_c.eos_batch(pr, P, T, x, role)
+    ok = (st == 0) & (ost == 0)
+    assert ok.mean() > 0.9
+    dz = np.abs(Z[ok] - oZ[ok])
+    dphi = np.abs(np.exp(lnphi[ok]) - np.exp(olnphi[ok])) / np.exp(olnphi[ok])
+    print(f"dry CPA vs PR: {int(ok.sum())} states, max |dZ| {dz.max():.2e}, max rel dphi {dphi.max():.2e}")
+    assert dz.max() <= 2e-8 and dphi.max() <= 3e-8
+
+
+@pytest.mark.gpu
+def test_gpu_cpa_single_donor_solver_matches_analytic_water():
+    """With CO2's acceptor site present but zero cross-association strength, the
+    single-donor Newton solver must reproduce the analytic 4C-water site fraction
+    X = 2 / (1 + sqrt(1 + 4A)) (cubic_eos.hpp:1606-1623 vs :1624-1730)."""
+    analytic = fluids.cpa_h2o_co2_nc10()
+    analytic.cpa_acceptors[1] = 0
+    analytic.cpa_epsilon[1] = analytic.cpa_beta[1] = 0.0
+    analytic.cpa_cross_epsilon[1] = analytic.cpa_cross_beta[1] = 0.0
+    newton = fluids.cpa_h2o_co2_nc10()
+    newton.cpa_epsilon[1] = newton.cpa_beta[1] = 0.0
+    newton.cpa_cross_epsilon[1] = newton.cpa_cross_beta[1] = 0.0
+    P, T, x, role = _cpa_states(256, 33)
+    Za, la, _, sa = (t.cpu().numpy() for t in _flash_gpu(analytic).eos_phase(P, T, x, role))
+    Zn, ln_, _, sn = (t.cpu().numpy() for t in _flash_gpu(newton).eos_phase(P, T, x, role))
+    assert np.array_equal(sa, sn)
+    ok = sa == 0
+    dz = np.abs(Za[ok] - Zn[ok]) / np.abs(Za[ok])
+    dl = np.abs(la[ok] - ln_[ok])
+    print(f"analytic vs single-donor: max rel dZ {dz.max():.2e}, max |dlnphi| {dl.max():.2e}")
+    assert dz.max() <= 1e-9 and dl.max() <= 1e-9
