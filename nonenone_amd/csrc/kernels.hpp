@@ -165,17 +165,47 @@ __global__ void __launch_bounds__(kBlock) flashKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
     const uint8_t *__restrict__ allowed, const double *__restrict__ seed, nn_flash_out out,
-    WorkQueue *__restrict__ q, unsigned int *__restrict__ list)
+    WorkQueue *__restrict__ q, unsigned int *__restrict__ list,
+    const unsigned int *__restrict__ order)
 {
-    // Persistent work queue: each work-item claims states until none are left.
+    // Persistent work queue: each work-item claims states until none are left,
+    // in `order` (states of similar conditions next to each other) when given.
     for (;;)
     {
-        const int64_t s = int64_t(atomicAdd(&q->head, 1u));
-        if (s >= n)
+        const int64_t i = int64_t(atomicAdd(&q->head, 1u));
+        if (i >= n)
             break;
+        const int64_t s = order ? int64_t(order[i]) : i;
         if (!flashOne<NC, kModeDefer>(*fp, *op, s, P, T, z, allowed, seed, out))
             deferItem(q, list, s);
     }
+}
+
+// Sort keys that put states of similar (T, P, z) next to each other, so that
+// the 64 states of a wavefront tend to follow the same iteration paths.
+__device__ inline unsigned int quantize(double v, double lo, double hi, int bits)
+{
+    const double top = double((1u << bits) - 1u);
+    double t = (v - lo) / (hi - lo);
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    return static_cast<unsigned int>(t * top);
+}
+
+template <int NC>
+__global__ void flashKeyKernel(int64_t n, const double *__restrict__ P,
+                               const double *__restrict__ T, const double *__restrict__ z,
+                               unsigned int *__restrict__ keys, unsigned int *__restrict__ idx)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const double lp = ::log(P[s] > 1.0 ? P[s] : 1.0);
+    unsigned int k = quantize(T[s], 250.0, 750.0, 10) << 22;
+    k |= quantize(lp, 11.5, 18.5, 10) << 12;
+    k |= quantize(z[s * NC], 0.0, 1.0, 6) << 6;
+    k |= quantize(z[s * NC + 1], 0.0, 1.0, 6);
+    keys[s] = k;
+    idx[s] = static_cast<unsigned int>(s);
 }
 
 // One wavefront per deferred state (blocks of 64 = one wave).
@@ -231,17 +261,47 @@ __global__ void __launch_bounds__(kBlock) phaseUpdateKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
     int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
-    unsigned int *__restrict__ list)
+    unsigned int *__restrict__ list, const unsigned int *__restrict__ order)
 {
     for (;;)
     {
-        const int64_t s = int64_t(atomicAdd(&q->head, 1u));
-        if (s >= n)
+        const int64_t i = int64_t(atomicAdd(&q->head, 1u));
+        if (i >= n)
             break;
+        const int64_t s = order ? int64_t(order[i]) : i;
         if (!phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary, phaseState, s,
                                                    status, flags))
             deferItem(q, list, s);
     }
+}
+
+// Phase-update sort key: presence, whether an active saturation sits at or
+// below the removal probe, then (T, P, water fraction of the oil slot).
+template <int NC, bool Well>
+__global__ void phaseUpdateKeyKernel(int64_t n, const double *__restrict__ T,
+                                     const double *__restrict__ primary,
+                                     const double *__restrict__ phaseState,
+                                     unsigned int *__restrict__ keys, unsigned int *__restrict__ idx)
+{
+    using L = Layout<NC, Well>;
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const double flag = phaseState[s * L::NPS + L::flag];
+    const unsigned int presence = static_cast<unsigned int>(flag > 0.0 && flag < 7.5 ? flag + 0.5 : 0.0) & 7u;
+    unsigned int removed = 0;
+    for (int p = 0; p < 3; ++p)
+        if ((presence & (1u << p)) && primary[s * L::NP + L::satBegin + p] <= kProbeSaturation)
+            removed = 1;
+    const double P = primary[s * L::NP + L::pressure];
+    const double lp = ::log(P > 1.0 ? P : 1.0);
+    unsigned int k = presence << 29;
+    k |= removed << 28;
+    k |= quantize(T[s], 250.0, 750.0, 10) << 18;
+    k |= quantize(lp, 11.5, 18.5, 10) << 8;
+    k |= quantize(primary[s * L::NP + L::compBegin(0)], 0.0, 1.0, 8);
+    keys[s] = k;
+    idx[s] = static_cast<unsigned int>(s);
 }
 
 template <int NC, bool Well>
@@ -474,10 +534,11 @@ hipError_t Launch<NC>::flash(unsigned grid, unsigned coopGrid, hipStream_t st,
                              const FluidParams *fp, const FlashOptions *op, int64_t n,
                              const double *P, const double *T, const double *z,
                              const uint8_t *allowed, const double *seed, nn_flash_out out,
-                             WorkQueue *q, unsigned int *list, hipEvent_t splitA,
-                             hipEvent_t splitB)
+                             WorkQueue *q, unsigned int *list, const unsigned int *order,
+                             hipEvent_t splitA, hipEvent_t splitB)
 {
-    flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, out, q, list);
+    flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, out, q, list,
+                                             order);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
@@ -486,6 +547,28 @@ hipError_t Launch<NC>::flash(unsigned grid, unsigned coopGrid, hipStream_t st,
     if (splitB && (e = hipEventRecord(splitB, st)) != hipSuccess)
         return e;
     flashCoopKernel<NC><<<coopGrid, kBlock, 0, st>>>(fp, op, P, T, z, allowed, seed, out, q, list);
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t Launch<NC>::flashKeys(hipStream_t st, int64_t n, const double *P, const double *T,
+                                 const double *z, unsigned int *keys, unsigned int *idx)
+{
+    const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+    flashKeyKernel<NC><<<grid, 256, 0, st>>>(n, P, T, z, keys, idx);
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t Launch<NC>::phaseUpdateKeys(bool well, hipStream_t st, int64_t n, const double *T,
+                                       const double *primary, const double *phaseState,
+                                       unsigned int *keys, unsigned int *idx)
+{
+    const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+    if (well)
+        phaseUpdateKeyKernel<NC, true><<<grid, 256, 0, st>>>(n, T, primary, phaseState, keys, idx);
+    else
+        phaseUpdateKeyKernel<NC, false><<<grid, 256, 0, st>>>(n, T, primary, phaseState, keys, idx);
     return hipGetLastError();
 }
 
@@ -507,15 +590,15 @@ hipError_t Launch<NC>::phaseUpdate(bool well, unsigned grid, unsigned coopGrid, 
                                    const FluidParams *fp, const FlashOptions *op, int64_t n,
                                    const double *T, double *primary, double *phaseState,
                                    int32_t *status, uint8_t *flags, WorkQueue *q,
-                                   unsigned int *list, hipEvent_t splitA,
-                             hipEvent_t splitB)
+                                   unsigned int *list, const unsigned int *order,
+                                   hipEvent_t splitA, hipEvent_t splitB)
 {
     if (well)
         phaseUpdateKernel<NC, true><<<grid, kBlock, 0, st>>>(fp, op, n, T, primary, phaseState,
-                                                             status, flags, q, list);
+                                                             status, flags, q, list, order);
     else
         phaseUpdateKernel<NC, false><<<grid, kBlock, 0, st>>>(fp, op, n, T, primary, phaseState,
-                                                              status, flags, q, list);
+                                                              status, flags, q, list, order);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;

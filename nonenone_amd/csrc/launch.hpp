@@ -52,8 +52,8 @@ struct Launch
                             const FluidParams *fp, const FlashOptions *op, int64_t n,
                             const double *P, const double *T, const double *z,
                             const uint8_t *allowed, const double *seed, nn_flash_out out,
-                            WorkQueue *q, unsigned int *list, hipEvent_t splitA,
-                            hipEvent_t splitB);
+                            WorkQueue *q, unsigned int *list, const unsigned int *order,
+                            hipEvent_t splitA, hipEvent_t splitB);
     static hipError_t stability(unsigned grid, hipStream_t st, const FluidParams *fp,
                                 const FlashOptions *op, int64_t n, const double *P,
                                 const double *T, const double *z, const uint8_t *active,
@@ -63,8 +63,14 @@ struct Launch
                                   const FluidParams *fp, const FlashOptions *op, int64_t n,
                                   const double *T, double *primary, double *phaseState,
                                   int32_t *status, uint8_t *flags, WorkQueue *q,
-                                  unsigned int *list, hipEvent_t splitA,
-                            hipEvent_t splitB);
+                                  unsigned int *list, const unsigned int *order,
+                                  hipEvent_t splitA, hipEvent_t splitB);
+    // Sort keys (and identity indices) for the processing order of a batch.
+    static hipError_t flashKeys(hipStream_t st, int64_t n, const double *P, const double *T,
+                                const double *z, unsigned int *keys, unsigned int *idx);
+    static hipError_t phaseUpdateKeys(bool well, hipStream_t st, int64_t n, const double *T,
+                                      const double *primary, const double *phaseState,
+                                      unsigned int *keys, unsigned int *idx);
     static hipError_t record(bool well, hipStream_t st, const FluidParams *fp, AssemblyView v,
                              const double *primary, const double *phaseState, double *accum,
                              int32_t *status);

@@ -5,6 +5,7 @@
 // chip pulls state indices from a device counter, so work-items that finish
 // early take new states instead of idling while slow states iterate.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -35,6 +36,13 @@ struct nn_context
     int numCUs = 256;
     int64_t maxBlocks = 0;  // NN_PERSISTENT_BLOCKS override of the persistent grid (0 = auto)
     int64_t coopBlocks = 0; // NN_COOP_BLOCKS override of the cooperative grid (0 = auto)
+    // Processing order of large batches (radix-sorted condition keys); NN_ORDER=0 disables.
+    bool ordering = true;
+    unsigned int *dKeys[2] = {nullptr, nullptr};
+    unsigned int *dIdx[2] = {nullptr, nullptr};
+    void *dSortTemp = nullptr;
+    size_t sortTempBytes = 0;
+    int64_t sortCap = 0;
     // Kernel timing: (start, end) event pairs per kernel kind while enabled.
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[NN_KERNEL_KINDS];
@@ -178,6 +186,59 @@ int ensureList(nn_context *ctx, int64_t n)
         return fail(NN_E_OUT_OF_MEMORY, "device allocation of the deferred list failed");
     ctx->listCap = n;
     return NN_OK;
+}
+
+constexpr int64_t kOrderMinItems = 4096;  // smaller batches run in index order
+
+void releaseSort(nn_context *ctx)
+{
+    for (int k = 0; k < 2; ++k)
+    {
+        if (ctx->dKeys[k]) (void)hipFree(ctx->dKeys[k]);
+        if (ctx->dIdx[k]) (void)hipFree(ctx->dIdx[k]);
+        ctx->dKeys[k] = ctx->dIdx[k] = nullptr;
+    }
+    if (ctx->dSortTemp) (void)hipFree(ctx->dSortTemp);
+    ctx->dSortTemp = nullptr;
+    ctx->sortTempBytes = 0;
+    ctx->sortCap = 0;
+}
+
+// Key/index buffers and radix-sort scratch for n items (synchronous when it grows).
+int ensureSort(nn_context *ctx, int64_t n)
+{
+    if (n <= ctx->sortCap)
+        return NN_OK;
+    releaseSort(ctx);
+    const size_t bytes = sizeof(unsigned int) * size_t(n);
+    for (int k = 0; k < 2; ++k)
+        if (hipMalloc(&ctx->dKeys[k], bytes) != hipSuccess || hipMalloc(&ctx->dIdx[k], bytes) != hipSuccess)
+        {
+            releaseSort(ctx);
+            return fail(NN_E_OUT_OF_MEMORY, "device allocation of the ordering buffers failed");
+        }
+    size_t temp = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, temp, ctx->dKeys[0], ctx->dKeys[1], ctx->dIdx[0],
+                                           ctx->dIdx[1], int(n)) != hipSuccess ||
+        hipMalloc(&ctx->dSortTemp, temp > 0 ? temp : 1) != hipSuccess)
+    {
+        releaseSort(ctx);
+        return fail(NN_E_OUT_OF_MEMORY, "device allocation of the sort scratch failed");
+    }
+    ctx->sortTempBytes = temp;
+    ctx->sortCap = n;
+    return NN_OK;
+}
+
+// Sorts the keys written into dKeys[0]/dIdx[0]; returns the ordered indices.
+hipError_t sortOrder(nn_context *ctx, int64_t n, hipStream_t st, const unsigned int **order)
+{
+    size_t temp = ctx->sortTempBytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(ctx->dSortTemp, temp, ctx->dKeys[0],
+                                                      ctx->dKeys[1], ctx->dIdx[0], ctx->dIdx[1],
+                                                      int(n), 0, 32, st);
+    *order = ctx->dIdx[1];
+    return e;
 }
 
 unsigned int coopGrid(const nn_context *ctx)
@@ -343,6 +404,8 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
         ctx->maxBlocks = std::max(0L, std::atol(env));
     if (const char *env = std::getenv("NN_COOP_BLOCKS"))
         ctx->coopBlocks = std::max(0L, std::atol(env));
+    if (const char *env = std::getenv("NN_ORDER"))
+        ctx->ordering = std::atoi(env) != 0;
     nn_flash_options defaults;
     nn_flash_options_default(&defaults);
     ctx->host = toParams(*fluid);
@@ -396,6 +459,7 @@ int nn_context_destroy(nn_context *ctx)
     if (ctx->dOpts) (void)hipFree(ctx->dOpts);
     if (ctx->dQueue) (void)hipFree(ctx->dQueue);
     if (ctx->dList) (void)hipFree(ctx->dList);
+    releaseSort(ctx);
     delete ctx;
     return NN_OK;
 }
@@ -438,17 +502,27 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
+    const bool ordered = ctx->ordering && n >= kOrderMinItems;
+    if (ordered)
+        if (const int rc = ensureSort(ctx, n); rc != NN_OK)
+            return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const nn_flash_out o = *out;
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
         resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue);
         NN_HIP(hipGetLastError());
+        const unsigned int *order = nullptr;
+        if (ordered)
+        {
+            NN_HIP((Launch<NC>::flashKeys(st, n, d_P, d_T, d_z, ctx->dKeys[0], ctx->dIdx[0])));
+            NN_HIP(sortOrder(ctx, n, st, &order));
+        }
         LaunchTimer t(ctx, NN_KERNEL_FLASH, st);
         t.makeSplit();
         NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), coopGrid(ctx), st, ctx->dFluid,
                                   ctx->dOpts, n, d_P, d_T, d_z, d_allowed, d_seed, o, ctx->dQueue,
-                                  ctx->dList, t.splitA, t.splitB)));
+                                  ctx->dList, order, t.splitA, t.splitB)));
         t.doneSplit(NN_KERNEL_FLASH_COOP);
         return NN_OK;
     });
@@ -510,17 +584,28 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
+    const bool ordered = ctx->ordering && n >= kOrderMinItems;
+    if (ordered)
+        if (const int rc = ensureSort(ctx, n); rc != NN_OK)
+            return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
         resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue);
         NN_HIP(hipGetLastError());
+        const unsigned int *order = nullptr;
+        if (ordered)
+        {
+            NN_HIP((Launch<NC>::phaseUpdateKeys(has_well_unknown != 0, st, n, d_T, d_primary,
+                                                d_phase_state, ctx->dKeys[0], ctx->dIdx[0])));
+            NN_HIP(sortOrder(ctx, n, st, &order));
+        }
         LaunchTimer t(ctx, NN_KERNEL_PHASE_UPDATE, st);
         t.makeSplit();
         NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, persistentGrid(ctx, n),
                                         coopGrid(ctx), st, ctx->dFluid, ctx->dOpts, n, d_T,
                                         d_primary, d_phase_state, d_status, d_flags, ctx->dQueue,
-                                        ctx->dList, t.splitA, t.splitB)));
+                                        ctx->dList, order, t.splitA, t.splitB)));
         t.doneSplit(NN_KERNEL_PHASE_UPDATE_COOP);
         return NN_OK;
     });
