@@ -372,9 +372,189 @@ __host__ __device__ double cpaDensityRootValue(const FluidParams &f, const CpaTe
     return 0.5 * (lo + hi);
 }
 
+// cpaDensityRootValue for one state held by a whole wavefront (every lane has
+// the same arguments): the same bracket points and bisection midpoints, with
+// the pressure evaluations of up to 64 bracket points, or of a 6-level
+// bisection tree (63 midpoints), done at once on separate lanes and then
+// consumed in the serial order with the serial stopping rules.  Points the
+// serial search would not have reached are evaluated but never used (their
+// errors included), so root and errors are those of the serial search.
+template <int NC>
+__host__ __device__ double cpaDensityRootValueCoop(const FluidParams &f, const CpaTemp<NC> &c,
+                                                   double P, double T, const double *x,
+                                                   bool liquid, const CpaMix<NC, double> &m,
+                                                   Err &err)
+{
+    const double b = m.b;
+    if (!(b > 0.0))
+    {
+        err.raise(kErrThermo);
+        return 0.0;
+    }
+    const double rhoMax = 0.999999 / b;
+    const int lane = waveLane();
+    auto evalAt = [&](double rho, double &fv, int &code) {
+        Err e;
+        fv = cpaPressure<NC, double>(f, c, rho, T, x, m, e) - P;
+        code = e.code;
+    };
+    double lo = 0.0, hi = 0.0, flo = -P, fhi = 0.0;
+    if (!liquid)
+    {
+        // points h_0 = min(rhoMax, max(1, P / (4RT))), h_{k+1} = min(0.999999 rhoMax, 1.6 h_k + 1)
+        double base = mind(rhoMax, maxd(1.0, 0.25 * P / (kGasConstant * T)));
+        bool first = true;
+        for (;;)
+        {
+            double h = base;
+            for (int j = 0; j < lane; ++j) h = mind(0.999999 * rhoMax, h * 1.6 + 1.0);
+            double fv;
+            int code;
+            evalAt(h, fv, code);
+            bool done = false;
+            double last = h;
+            for (int j = 0; j < 64 && !done; ++j)
+            {
+                const int e = waveShfl(code, j);
+                if (e != kErrNone)
+                {
+                    err.raise(e);
+                    return 0.0;
+                }
+                const double hj = waveShfl(h, j), fj = waveShfl(fv, j);
+                if (!first)
+                {
+                    lo = hi;
+                    flo = fhi;
+                }
+                first = false;
+                hi = hj;
+                fhi = fj;
+                last = hj;
+                done = !(fhi < 0.0 && hi < 0.999 * rhoMax);
+            }
+            if (done)
+                break;
+            base = mind(0.999999 * rhoMax, last * 1.6 + 1.0);
+        }
+    }
+    else
+    {
+        // points rhoMax, 0.82 rhoMax, then repeated * 0.82; at most 160 shrink steps
+        double start = rhoMax;
+        int consumed = 0;  // points used so far (0 = none)
+        int shrinks = 0;
+        bool done = false;
+        while (!done)
+        {
+            double pnt = start;
+            for (int j = 0; j < lane; ++j) pnt = (consumed + j == 0) ? 0.82 * pnt : pnt * 0.82;
+            double fv;
+            int code;
+            evalAt(pnt, fv, code);
+            double last = pnt;
+            for (int j = 0; j < 64 && !done; ++j)
+            {
+                const int e = waveShfl(code, j);
+                if (e != kErrNone)
+                {
+                    err.raise(e);
+                    return 0.0;
+                }
+                const double pj = waveShfl(pnt, j), fj = waveShfl(fv, j);
+                last = pj;
+                const int index = consumed + j;
+                if (index == 0)
+                {
+                    hi = pj;
+                    fhi = fj;
+                }
+                else if (index == 1)
+                {
+                    lo = pj;
+                    flo = fj;
+                    done = !(shrinks < 160 && flo > 0.0);
+                }
+                else
+                {
+                    hi = lo;
+                    fhi = flo;
+                    lo = pj;
+                    flo = fj;
+                    ++shrinks;
+                    done = !(shrinks < 160 && flo > 0.0);
+                }
+            }
+            if (done)
+                break;
+            consumed += 64;
+            start = (consumed == 0) ? 0.82 * last : last * 0.82;
+        }
+    }
+    if (!(flo <= 0.0 && fhi >= 0.0))
+    {
+        err.raise(kErrThermo);
+        return 0.0;
+    }
+    // bisection, six levels per round: lane j < 63 evaluates heap node j
+    int it = 0;
+    bool stop = false;
+    while (it < 100 && !stop)
+    {
+        double a = lo, bb = hi;
+        double fm = 0.0;
+        int code = kErrNone;
+        if (lane < 63)
+        {
+            const unsigned node = unsigned(lane) + 1u;  // 1-based heap index
+            int depth = 0;
+            while ((node >> (depth + 1)) != 0u) ++depth;
+            for (int d = depth - 1; d >= 0; --d)
+            {
+                const double mid = 0.5 * (a + bb);
+                if ((node >> d) & 1u)
+                    a = mid;  // right child: f(mid) <= 0 moved lo up
+                else
+                    bb = mid;  // left child: f(mid) > 0 moved hi down
+            }
+            evalAt(0.5 * (a + bb), fm, code);
+        }
+        int node = 0;
+        for (int level = 0; level < 6 && it < 100; ++level)
+        {
+            const double mid = 0.5 * (lo + hi);
+            const int e = waveShfl(code, node);
+            if (e != kErrNone)
+            {
+                err.raise(e);
+                return 0.0;
+            }
+            const double fmid = waveShfl(fm, node);
+            if (fmid > 0.0)
+            {
+                hi = mid;
+                fhi = fmid;
+            }
+            else
+            {
+                lo = mid;
+                flo = fmid;
+            }
+            ++it;
+            if (::fabs(fmid) <= 1.0e-10 * maxd(P, 1.0) || (hi - lo) <= 1.0e-12 * maxd(mid, 1.0))
+            {
+                stop = true;
+                break;
+            }
+            node = fmid > 0.0 ? 2 * node + 1 : 2 * node + 2;
+        }
+    }
+    return 0.5 * (lo + hi);
+}
+
 // :2191-2248: scalar root, then two implicit-function corrections
 // rho -= (p(rho) - P) / (dp/drho)_FD, which also carry first derivatives.
-template <int NC, class S>
+template <int NC, class S, bool Coop = false>
 __host__ __device__ S cpaDensityRoot(const FluidParams &f, const CpaTemp<NC> &c, const S &P,
                                      const S &T, const S *x, bool liquid, Err &err)
 {
@@ -383,7 +563,8 @@ __host__ __device__ S cpaDensityRoot(const FluidParams &f, const CpaTemp<NC> &c,
     const double pV = sval(P), tV = sval(T);
     CpaMix<NC, double> sm;
     cpaMixing<NC, double>(c, xs, sm);
-    const double root = cpaDensityRootValue<NC>(f, c, pV, tV, xs, liquid, sm, err);
+    const double root = Coop ? cpaDensityRootValueCoop<NC>(f, c, pV, tV, xs, liquid, sm, err)
+                             : cpaDensityRootValue<NC>(f, c, pV, tV, xs, liquid, sm, err);
     S rho = P;
     if (err.bad())
         return rho;
@@ -468,22 +649,22 @@ __host__ __device__ void cpaAtDensity(const FluidParams &f, const CpaTemp<NC> &c
 
 // :2309-2340 without the Gibbs-minimum root option: the selected root's Z and
 // ln(phi).  liquid = role is not gas.
-template <int NC, class S>
+template <int NC, class S, bool Coop = false>
 __host__ __device__ void cpaPhase(const FluidParams &f, const CpaTemp<NC> &c, const S &P,
                                   const S &T, const S *x, bool liquid, S &Z, S *lnPhi, Err &err)
 {
-    const S rho = cpaDensityRoot<NC, S>(f, c, P, T, x, liquid, err);
+    const S rho = cpaDensityRoot<NC, S, Coop>(f, c, P, T, x, liquid, err);
     if (err.bad())
         return;
     cpaAtDensity<NC, S>(f, c, P, T, x, rho, Z, lnPhi, err);
 }
 
 // :864-882 compressibility(): Z of the selected root only.
-template <int NC>
+template <int NC, bool Coop = false>
 __host__ __device__ double cpaCompressibility(const FluidParams &f, const CpaTemp<NC> &c, double P,
                                               double T, const double *x, bool liquid, Err &err)
 {
-    const double rho = cpaDensityRoot<NC, double>(f, c, P, T, x, liquid, err);
+    const double rho = cpaDensityRoot<NC, double, Coop>(f, c, P, T, x, liquid, err);
     if (err.bad())
         return 1.0;
     return P / (rho * kGasConstant * T);

@@ -210,7 +210,8 @@ public:
             if (cache.cpa.T != T)
                 cpaTemperature<NC>(f, T, cache.cpa);
             double lnPhi[NC];
-            cpaPhase<NC, double>(f, cache.cpa, P, T, x, role != kGas, out.Z, lnPhi, err);
+            cpaPhase<NC, double, Mode == kModeCoop>(f, cache.cpa, P, T, x, role != kGas, out.Z, lnPhi,
+                                                    err);
             for (int i = 0; i < NC; ++i) out.phi[i] = err.bad() ? 1.0 : ::exp(lnPhi[i]);
             return;
         }

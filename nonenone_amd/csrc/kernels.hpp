@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(kBlock) flashKernel(
         if (i >= n)
             break;
         const int64_t s = order ? int64_t(order[i]) : i;
-        if (!flashOne<NC, kModeDefer>(*fp, *op, s, P, T, z, allowed, seed, out))
+        if (q->deferAll || !flashOne<NC, kModeDefer>(*fp, *op, s, P, T, z, allowed, seed, out))
             deferItem(q, list, s);
     }
 }
@@ -269,8 +269,8 @@ __global__ void __launch_bounds__(kBlock) phaseUpdateKernel(
         if (i >= n)
             break;
         const int64_t s = order ? int64_t(order[i]) : i;
-        if (!phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary, phaseState, s,
-                                                   status, flags))
+        if (q->deferAll || !phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary,
+                                                                  phaseState, s, status, flags))
             deferItem(q, list, s);
     }
 }

@@ -37,7 +37,7 @@ struct WorkQueue
     unsigned int head;      // next item of the lane-per-item pass
     unsigned int deferred;  // items handed to the cooperative pass (= list length)
     unsigned int coopHead;  // next list entry of the cooperative pass
-    unsigned int pad;
+    unsigned int deferAll;  // nonzero: every item goes to the cooperative pass (testing)
 };
 
 template <int NC>

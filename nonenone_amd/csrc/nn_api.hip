@@ -36,6 +36,7 @@ struct nn_context
     int numCUs = 256;
     int64_t maxBlocks = 0;  // NN_PERSISTENT_BLOCKS override of the persistent grid (0 = auto)
     int64_t coopBlocks = 0; // NN_COOP_BLOCKS override of the cooperative grid (0 = auto)
+    bool coopAll = false;   // NN_COOP_ALL=1: every item through the cooperative pass (testing)
     // Processing order of large batches (radix-sorted condition keys); NN_ORDER=0 disables.
     bool ordering = true;
     unsigned int *dKeys[2] = {nullptr, nullptr};
@@ -88,12 +89,12 @@ int dispatchNc(int nc, F &&f)
 
 // Work-queue counters reset by a kernel rather than hipMemsetAsync, so that
 // the launch sequence stays valid inside a captured HIP graph.
-__global__ void resetQueueKernel(WorkQueue *q)
+__global__ void resetQueueKernel(WorkQueue *q, unsigned int deferAll)
 {
     q->head = 0u;
     q->deferred = 0u;
     q->coopHead = 0u;
-    q->pad = 0u;
+    q->deferAll = deferAll;
 }
 
 void releaseEvents(nn_context *ctx)
@@ -406,6 +407,8 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
         ctx->coopBlocks = std::max(0L, std::atol(env));
     if (const char *env = std::getenv("NN_ORDER"))
         ctx->ordering = std::atoi(env) != 0;
+    if (const char *env = std::getenv("NN_COOP_ALL"))
+        ctx->coopAll = std::atoi(env) != 0;
     nn_flash_options defaults;
     nn_flash_options_default(&defaults);
     ctx->host = toParams(*fluid);
@@ -510,7 +513,7 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
     const nn_flash_out o = *out;
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue);
+        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue, ctx->coopAll ? 1u : 0u);
         NN_HIP(hipGetLastError());
         const unsigned int *order = nullptr;
         if (ordered)
@@ -591,7 +594,7 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
     hipStream_t st = static_cast<hipStream_t>(stream);
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue);
+        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue, ctx->coopAll ? 1u : 0u);
         NN_HIP(hipGetLastError());
         const unsigned int *order = nullptr;
         if (ordered)

@@ -280,7 +280,7 @@ struct PhaseUpdater
             if (cpa)
             {
                 // :188-200: eos.compressibility (CPA density root)
-                Z = cpaCompressibility<NC>(f, cpaT, P, T, comp[p], p != kGas, err);
+                Z = cpaCompressibility<NC, Mode == kModeCoop>(f, cpaT, P, T, comp[p], p != kGas, err);
             }
             else
             {

@@ -147,3 +147,60 @@ def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     assert re.max() <= assemblycmp.REL_TOL
     assert je.max() <= assemblycmp.REL_TOL
     assert assemblycmp.structural_zero_mismatch(out["bsr"].cpu().numpy(), r_bsr) == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+def test_gpu_newton_assembly_step_replays_in_a_hip_graph():
+    """One Newton assembly (state restore, phase update, residual, BSR Jacobian) captured
+    once in a HIP graph and replayed three times, with eager work and synchronisations in
+    between: every replay must equal the eager step bit for bit."""
+    import torch
+    from nonenone_amd.assembly import NaturalAssembly
+    from nonenone_amd.flash import ThreePhaseFlash
+
+    fluid = fluids.h2o_co2_nc10("PR")
+    ref = refharness.Reference(fluid, FlashOptions.defaults())
+    w, hist_p, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 80, 64, 23, ref,
+                                                                has_well=True)
+    ctx = ThreePhaseFlash(fluid)
+    asm = NaturalAssembly(ctx, w["topo"], w["volume"], w["porosity"], w["T"], True)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    T = dev(w["T"])
+    acc = asm.accumulation(dev(hist_p), dev(hist_ps))
+    p0, s0 = dev(cur), dev(hist_ps)
+    prim, ps = p0.clone(), s0.clone()
+    out = asm.allocate()
+
+    def step():
+        prim.copy_(p0)
+        ps.copy_(s0)
+        st, _ = ctx.update_phase_state(T, prim, ps, True)
+        asm.evaluate(DT, prim, ps, acc, out=out)
+        return st
+
+    st_eager = step()                      # also sizes the library's work buffers
+    torch.cuda.synchronize()
+    eager = {k: v.clone() for k, v in out.items()}
+    eager_prim, eager_status = prim.clone(), st_eager.clone()
+
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        step()                             # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(stream)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        st_graph = step()
+    for replay in range(3):
+        for v in out.values():
+            v.zero_()
+        prim.fill_(0.0)
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(prim, eager_prim), f"replay {replay}: primary"
+        assert torch.equal(st_graph, eager_status), f"replay {replay}: status"
+        for k in ("residual", "bsr", "decisions", "status"):
+            assert torch.equal(out[k], eager[k]), f"replay {replay}: {k}"
+        _ = (prim * 2.0).sum().item()      # eager work between replays

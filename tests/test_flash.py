@@ -279,3 +279,23 @@ def test_gpu_cpa_single_donor_solver_matches_analytic_water():
     dl = np.abs(la[ok] - ln_[ok])
     print(f"analytic vs single-donor: max rel dZ {dz.max():.2e}, max |dlnphi| {dl.max():.2e}")
     assert dz.max() <= 1e-9 and dl.max() <= 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["flash_nc3_pr", "flash_nc3_cpa", "flash_nc3_sw"])
+def test_gpu_cooperative_pass_matches_golden(name, monkeypatch):
+    """Every state through the wavefront-per-state pass (NN_COOP_ALL=1): lane-parallel
+    RR3 guesses, allocation seeds and CPA density-root brackets / bisection trees
+    must reproduce the reference's outputs as the lane-per-state pass does."""
+    monkeypatch.setenv("NN_COOP_ALL", "1")
+    g = load_case(name)
+    fl = _flash_gpu(CASES[name]())
+    mine = flashcmp.to_numpy(fl.flash(g["P"], g["T"], g["z"]))
+    n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, g)
+    unexplained, conditioned = libm_conditioning.split_value_failures(
+        flashcmp.per_state_value_diff(mine, g), sel, flashcmp.REL_TOL, name)
+    print(f"{name} (cooperative pass): status mismatches {n_status}, label mismatches {n_label}, "
+          f"max rel diff {worst:.3e}; above tolerance: {len(conditioned)} libm-conditioned, "
+          f"{len(unexplained)} other {unexplained}")
+    assert n_status == 0 and n_label == 0
+    assert not unexplained
