@@ -26,6 +26,27 @@ device copy (part of the timed work) because the phase update is in place.
 Prints one JSON line.  The `cpu_baseline` leg runs the reference's own code
 (the oracle's build of it, oracle/_ref) on a 50 x 50 x 50 sub-block of the
 same workload with one thread per host core.
+
+Other workloads (same JSON contract, for the figures quoted beside the headline):
+
+    --workload flash    batched three-phase flash of 2^20 independent states
+                        (half nc=3 H2O/CO2/nC10, half nc=6 H2O/CO2/C1/nC4/nC10/nC16,
+                        each half split evenly between PR78 and Soreide-Whitson;
+                        P ~ U(1, 40) MPa, T ~ U(300, 600) K, z ~ Dirichlet(1..1), 10 %
+                        of states within 2 % of a Kay's-rule pseudo-critical point),
+                        with launch shapes picked by the autotuner on an 8192-state
+                        sample of each sub-batch;
+                        also times the plain one-kernel form on the same states.
+    --workload serving  64 ensemble members (nc=3/5/6/8, PR/SW/PR-CPA, 1k-64k cells each)
+                        submit 256 Newton-step requests with Poisson arrivals (200/s);
+                        continuous batching on 8 stream slots vs the plain
+                        one-request-at-a-time form on the same trace.
+    --workload learned  the learned initialiser: trains the MLP on flash results
+                        of 4M nc=6 PR states of the batched-flash distribution (FP32,
+                        the library's own kernels), then flashes a fresh 2^20-state
+                        sample with and without its hints (BF16 inference,
+                        confidence threshold calibrated for zero label changes on
+                        a 65536-state validation sample).
 """
 from __future__ import annotations
 
@@ -157,6 +178,281 @@ def cpu_baseline(fluid, host, dims, sub=(50, 50, 50), target_s=12.0):
                       f"of phase update + residual + BSR Jacobian in {t_total:.1f}s"}
 
 
+FLASH_CASES = (("nc3", "PR"), ("nc3", "SW"), ("nc6", "PR"), ("nc6", "SW"))
+
+
+def flash_fluid(case):
+    from nonenone_amd import fluids
+    comp, model = case
+    return fluids.h2o_co2_nc10(model) if comp == "nc3" else fluids.six_component(model)
+
+
+def flash_bytes(nc: int) -> int:
+    """Compulsory HBM bytes of one flash: read P, T, z; write presence, converged,
+    status, iterations, beta, S, x, Z, rho_m, K_go, K_wo."""
+    return (2 + nc) * 8 + 1 + 1 + 4 + 4 + 3 * 8 * 4 + 3 * nc * 8 + 2 * nc * 8
+
+
+def flash_workload(args) -> dict:
+    """2^20 independent states (4 sub-batches) through nn_flash_batch."""
+    import torch
+
+    from nonenone_amd import workloads
+    from nonenone_amd.flash import FlashBatchResult, ThreePhaseFlash
+
+    total = args.states
+    per = total // len(FLASH_CASES)
+    subs = []
+    for k, case in enumerate(FLASH_CASES):
+        fluid = flash_fluid(case)
+        fl = ThreePhaseFlash(fluid)
+        P, T, z = workloads.batched_flash_states(per, fluid, seed=100 + k,
+                                                 near_critical_fraction=0.1)
+        d = dict(P=torch.as_tensor(P, device="cuda"), T=torch.as_tensor(T, device="cuda"),
+                 z=torch.as_tensor(z, device="cuda"))
+        t0 = time.time()
+        sample = min(per, args.tune_sample)
+        best, best_ms = fl.autotune(d["P"][:sample], d["T"][:sample], d["z"][:sample], repeats=2)
+        log(f"{case}: autotune on {sample} states {time.time() - t0:.1f}s -> {best} "
+            f"({best_ms:.1f} ms)")
+        subs.append(dict(case=case, fl=fl, d=d, host=(P, T, z), sched=best,
+                         out=FlashBatchResult.empty(per, fl.nc)))
+
+    def step():
+        for sb in subs:
+            sb["fl"].flash(sb["d"]["P"], sb["d"]["T"], sb["d"]["z"], out=sb["out"])
+
+    for i in range(args.warmup):
+        t0 = time.time()
+        step()
+        torch.cuda.synchronize()
+        log(f"warmup {i}: {time.time() - t0:.3f}s")
+    for sb in subs:
+        sb["fl"].set_kernel_timing(True)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        log(f"step {i}: {time.perf_counter() - t0:.3f}s since the timed region began")
+    elapsed = time.perf_counter() - t0
+    kern = {}
+    for sb in subs:
+        ms, cnt = sb["fl"].kernel_time("flash")
+        cms, _ = sb["fl"].kernel_time("flash_coop")
+        kern["/".join(sb["case"])] = dict(flash_ms=ms / max(cnt, 1), coop_ms=cms / max(cnt, 1),
+                                          schedule=sb["sched"])
+        sb["fl"].set_kernel_timing(False)
+    nonok = sum(int((sb["out"].status != 0).sum().item()) for sb in subs)
+    log(f"timed {args.steps} steps in {elapsed:.3f}s; non-OK statuses {nonok}")
+
+    # the plain one-kernel form on the same states (one pass)
+    plain = {}
+    t_plain = 0.0
+    for sb in subs:
+        fl = sb["fl"]
+        keep = fl.schedule()
+        fl.set_schedule(plain=1)
+        ref_out = FlashBatchResult.empty(per, fl.nc)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        fl.flash(sb["d"]["P"], sb["d"]["T"], sb["d"]["z"], out=ref_out)
+        torch.cuda.synchronize()
+        dt_ = time.perf_counter() - ts
+        t_plain += dt_
+        fl.set_schedule(**{k: v for k, v in keep.items()})
+        same = all(torch.equal(getattr(ref_out, f), getattr(sb["out"], f))
+                   for f in ("presence", "status", "beta", "x", "Z"))
+        plain["/".join(sb["case"])] = dict(seconds=dt_, bit_identical=same)
+        log(f"plain form {sb['case']}: {dt_:.3f}s, results bit-identical: {same}")
+    per_step = elapsed / args.steps
+    abytes = sum(per * flash_bytes(sb["fl"].nc) for sb in subs)
+    achieved = abytes / per_step / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = flash_cpu_baseline(subs, args.cpu_sample)
+        log(f"cpu baseline: {cpu}")
+    return {
+        "metric": "batched three-phase flashes/s (FP64)", "value": total * args.steps / elapsed,
+        "unit": "flashes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": per_step * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched flash of {total} states: nc=3 H2O/CO2/nC10 and nc=6 "
+                               "H2O/CO2/C1/nC4/nC10/nC16, PR78 and Soreide-Whitson, "
+                               "P~U(1,40) MPa, T~U(300,600) K, z~Dirichlet(1..1), 10% "
+                               "near-critical", "states": total},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "kernels_ms": kern,
+        "plain_form": {"seconds": t_plain, "flashes_per_s": total / t_plain,
+                       "speedup_of_scheduled": (total * args.steps / elapsed) / (total / t_plain),
+                       "per_case": plain},
+        "roofline": {"kernel": "flashKernel (all four sub-batches)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": abytes,
+                     "note": "FP64-VALU and divergence bound; HBM bytes are the compulsory "
+                             "inputs/outputs only"},
+        "cpu_baseline": cpu,
+    }
+
+
+def flash_cpu_baseline(subs, sample: int) -> dict | None:
+    """The reference's CubicThreePhaseFlash::flash (oracle/_ref) on the first `sample`
+    states of every sub-batch, one thread per host core."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import refharness
+
+    from nonenone_amd._abi import FlashOptions
+    if not refharness.available():
+        log("oracle/_ref/libnn_ref.so not built: no CPU baseline")
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    t = 0.0
+    n = 0
+    for sb in subs:
+        ref = refharness.Reference(flash_fluid(sb["case"]), FlashOptions.defaults())
+        P, T, z = (a[:sample] for a in sb["host"])
+        t0 = time.perf_counter()
+        ref.flash(P, T, z, threads=cores)
+        t += time.perf_counter() - t0
+        n += len(P)
+    return {"value": n / t, "unit": "flashes/s", "cores": cores, "kind": "reference",
+            "sample": f"first {sample} states of each of the 4 sub-batches ({n} flashes) "
+                      f"in {t:.1f}s"}
+
+
+def learned_workload(args) -> dict:
+    """Train the learned initialiser on flash results, then flash a fresh sample with
+    and without its hints."""
+    import torch
+
+    from nonenone_amd import fluids, learned, workloads
+    from nonenone_amd.flash import ThreePhaseFlash
+
+    fluid = fluids.six_component("PR")
+    fl = ThreePhaseFlash(fluid)
+    model = learned.PhaseInitModel(6, seed=7)
+    t0 = time.time()
+    data = learned.generate(fl, model, args.train_states, seed=10_000)
+    torch.cuda.synchronize()
+    t_gen = time.time() - t0
+    log(f"training set: {data.X.shape[0]} converged flashes in {t_gen:.1f}s")
+    t0 = time.time()
+    hist = learned.train(model, data, epochs=args.epochs, batch=16384, lr=3e-3, log=log)
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    acc = learned.accuracy(model, data)
+    log(f"trained {args.epochs} epochs in {t_train:.1f}s; training accuracy {acc:.4f}")
+    Pv, Tv, zv = workloads.batched_flash_states(1 << 16, fluid, seed=77_777,
+                                                near_critical_fraction=0.1)
+    P, T, z = workloads.batched_flash_states(args.states, fluid, seed=99_999,
+                                             near_critical_fraction=0.1)
+    fl.flash(P[:4096], T[:4096], z[:4096])  # warm-up of the flash kernels
+    results = {}
+    for name, modes in (("single_phase", 1), ("log_k", 2), ("both", 3)):
+        th, rec = learned.calibrate(fl, model, Pv, Tv, zv, modes)
+        log(f"hints {name}: calibrated threshold {th} "
+            f"({ {k: v['label_mismatches'] for k, v in rec.items()} } label changes)")
+        if th is None:
+            results[name] = {"threshold": None}
+            continue
+        cmp, _, _ = learned.evaluate(fl, model, P, T, z, threshold=th, modes=modes,
+                                     repeats=max(1, args.steps))
+        cmp["threshold"] = th
+        results[name] = cmp
+        log(f"hints {name}: {cmp}")
+    scored = [r for r in results.values() if r.get("threshold") is not None]
+    best = max(scored, key=lambda r: r["speedup"] if r["label_mismatches"] == 0 else 0) \
+        if scored else None
+    if best is None:
+        raise RuntimeError("no hint mode passed calibration")
+    n = args.states
+    return {
+        "metric": "hinted batched flashes/s (FP64, zero label mismatches)",
+        "value": n / best["seeded_s"], "unit": "flashes/s", "n_gpus": 1,
+        "steps": max(1, args.steps), "warmup": 1, "ms_per_step": best["seeded_s"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"learned initialiser: MLP 8-64-64-64-9 SiLU trained (FP32) on "
+                               f"{data.X.shape[0]} flash results of nc=6 PR states, BF16 "
+                               f"inference, confidence threshold calibrated on 65536 "
+                               f"validation states, {n} fresh states",
+                   "states": n},
+        "training": {"flashes": int(data.X.shape[0]), "generate_s": t_gen, "train_s": t_train,
+                     "epochs": args.epochs, "loss": hist, "accuracy": acc},
+        "hints": results,
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+SERVING_KINDS = ("nc3-PR", "nc5-SW", "nc6-PR", "nc8-PR", "nc3-CPA", "nc3-SW", "nc5-PR",
+                 "nc6-SW", "nc8-SW")
+
+
+def serving_workload(args) -> dict:
+    """Ensemble members on one GPU: Poisson step requests, continuous batching vs the
+    plain one-request-at-a-time form on the same arrival trace."""
+    from nonenone_amd.serving import Member, Server, poisson_arrivals, run_trace
+
+    rng = np.random.Generator(np.random.PCG64(2024))
+    members = []
+    t0 = time.time()
+    for i in range(args.members):
+        kind = SERVING_KINDS[i % len(SERVING_KINDS)]
+        hi = args.max_member_cells if kind != "nc3-CPA" else min(args.max_member_cells, 8192)
+        cells = int(np.exp(rng.uniform(np.log(1000), np.log(hi))))
+        nx = 50
+        ny = max(1, (cells + nx - 1) // nx)
+        members.append(Member(kind, nx, ny, seed=7000 + i))
+    total_cells = sum(m.ncell for m in members)
+    log(f"{len(members)} members, {total_cells} cells in total ({time.time() - t0:.1f}s set-up)")
+    arrivals = poisson_arrivals(args.requests, len(members), rate=args.rate, seed=11)
+    budget = int(args.budget_gb * (1 << 30))
+    # warm-up: every member once (first launches allocate each context's queues)
+    run_trace(Server(args.slots, budget), members, [(0.0, i) for i in range(len(members))])
+    plain = run_trace(Server(1, budget), members, arrivals)
+    log(f"plain form: {plain.cells} cell-steps in {plain.makespan_s:.3f}s, "
+        f"p50 {np.percentile(plain.latencies_ms, 50):.1f} ms, "
+        f"p99 {np.percentile(plain.latencies_ms, 99):.1f} ms")
+    cont = run_trace(Server(args.slots, budget), members, arrivals)
+    same = dict(plain.checksums) == dict(cont.checksums)
+    log(f"continuous batching ({args.slots} slots): {cont.cells} cell-steps in "
+        f"{cont.makespan_s:.3f}s, p50 {np.percentile(cont.latencies_ms, 50):.1f} ms, "
+        f"p99 {np.percentile(cont.latencies_ms, 99):.1f} ms; stats {cont.stats}; "
+        f"results identical to the plain form: {same}")
+    value = cont.cells / cont.makespan_s
+    plain_rate = plain.cells / plain.makespan_s
+    return {
+        "metric": "served cell flash+Jacobian assemblies/s (FP64); p50/p99 request latency",
+        "value": value, "unit": "cells/s", "n_gpus": 1, "steps": len(arrivals), "warmup": 1,
+        "ms_per_step": cont.makespan_s * 1e3 / len(arrivals), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serving: {len(members)} members ({', '.join(SERVING_KINDS)}; "
+                               f"1k-{args.max_member_cells} cells each, CPA members <= 8k), "
+                               f"{len(arrivals)} Newton-step requests (phase update + residual "
+                               f"+ BSR Jacobian), Poisson arrivals at {args.rate}/s, "
+                               f"{args.slots} stream slots, {args.budget_gb} GB output budget",
+                   "members": len(members), "member_cells": total_cells},
+        "p50_ms": float(np.percentile(cont.latencies_ms, 50)),
+        "p99_ms": float(np.percentile(cont.latencies_ms, 99)),
+        "plain_form": {"value": plain_rate, "p50_ms": float(np.percentile(plain.latencies_ms, 50)),
+                       "p99_ms": float(np.percentile(plain.latencies_ms, 99)),
+                       "makespan_s": plain.makespan_s},
+        "speedup_vs_plain": value / plain_rate,
+        "results_identical_to_plain": same,
+        "server_stats": cont.stats,
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +463,19 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=50)
     ap.add_argument("--eos", choices=("CPA", "PR", "SW"), default="CPA")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("assembly", "flash", "learned", "serving"),
+                    default="assembly")
+    ap.add_argument("--tune-sample", type=int, default=8192)
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--max-member-cells", type=int, default=65536)
+    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--states", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--train-states", type=int, default=4 << 20)
+    ap.add_argument("--epochs", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the path is per-cell and shards by slabs of the grid "
@@ -178,6 +487,11 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         return 1
+    if args.workload != "assembly":
+        line = {"flash": flash_workload, "learned": learned_workload,
+                "serving": serving_workload}[args.workload](args)
+        print(json.dumps(line), flush=True)
+        return 0
     fluid, fl, asm, s, host = build_workload(args.nx, args.ny, args.nz, eos=args.eos)
     ncell, nconn, nnzb = asm.ncell, asm.nconn, asm.nnzb
     for i in range(args.warmup):

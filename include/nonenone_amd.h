@@ -177,6 +177,58 @@ int nn_flash_batch(nn_context *ctx, int64_t n,
                    const double *d_P, const double *d_T, const double *d_z,
                    const nn_flash_out *out, void *stream);
 
+/*
+ * F-seed: nn_flash_batch with a per-state hint from a learned initialiser (no
+ * counterpart in the reference).  d_hint[i]: NN_HINT_NONE runs the reference
+ * path unchanged; NN_HINT_SINGLE_PHASE skips the first gas-oil two-phase start
+ * of three_phase_flash.hpp:93-101 and continues from the single-phase candidate;
+ * NN_HINT_LOG_K replaces that start's Wilson K by exp(d_log_k[i][:]) (n x nc,
+ * may be NULL when no state carries NN_HINT_LOG_K).  Every later stage (active-set
+ * equilibration with its stability tests, three-phase flash, recovery chain) is
+ * the reference's, so the stability tests accept or overrule the hint.  Ordered
+ * batches sort by hint first, so states on the same solver path share wavefronts.
+ */
+#define NN_HINT_NONE 0
+#define NN_HINT_SINGLE_PHASE 1
+#define NN_HINT_LOG_K 2
+int nn_flash_seeded_batch(nn_context *ctx, int64_t n,
+                          const double *d_P, const double *d_T, const double *d_z,
+                          const uint8_t *d_hint, const double *d_log_k,
+                          const nn_flash_out *out, void *stream);
+
+/*
+ * Launch shapes of the flash-like batches of a context (no counterpart in the
+ * reference: how the per-state work maps onto the GPU).  Defaults {16, 4, 1, 0, 0}.
+ *   persistent_blocks_per_cu  grid of the lane-per-state pass (64-thread blocks
+ *                             per CU, capped by the batch), 1..64
+ *   coop_blocks_per_cu        grid of the wavefront-per-deferred-state pass, 1..64
+ *   ordered                   process batches of >= 4096 states in radix-sorted
+ *                             condition-key order
+ *   wave_claim                0: lanes claim single states from the queue;
+ *                             1: wavefronts claim tiles of 64 consecutive states
+ *   plain                     1: the baseline form for flash batches: one kernel,
+ *                             one work-item per state in index order, no queue,
+ *                             no deferral (other fields ignored)
+ */
+typedef struct nn_schedule {
+    int32_t persistent_blocks_per_cu;
+    int32_t coop_blocks_per_cu;
+    int32_t ordered;
+    int32_t wave_claim;
+    int32_t plain;
+} nn_schedule;
+int nn_context_get_schedule(nn_context *ctx, nn_schedule *s);
+int nn_context_set_schedule(nn_context *ctx, const nn_schedule *s);
+
+/* Autotuner: times nn_flash_batch on the given sample (device arrays, outputs
+ * overwritten) under each candidate schedule (ordered x wave_claim x
+ * persistent_blocks_per_cu in {4, 8, 16, 32}), best of `repeats` runs each,
+ * keeps the fastest in the context and returns it with its time.  Synchronous. */
+int nn_flash_autotune(nn_context *ctx, int64_t n,
+                      const double *d_P, const double *d_T, const double *d_z,
+                      const nn_flash_out *out, int32_t repeats,
+                      nn_schedule *best, double *best_ms, void *stream);
+
 /* F-restricted: CubicThreePhaseFlash::flashRestricted(P,T,z,allowed[,seed])
  * (three_phase_flash.hpp:235-292).  d_allowed: n presence masks; d_seed may be
  * NULL (no composition seed) or n x 3 x nc. */
@@ -264,6 +316,107 @@ int nn_assembly_evaluate(nn_assembly *a, double dt, double fugacity_scaling,
                          const double *d_accum_prev, double *d_residual,
                          double *d_bsr, uint8_t *d_decisions,
                          int32_t *d_status, void *stream);
+
+/* Sizes of an assembly handle: its context, cells, BSR blocks and primary
+ * variables per cell (no counterpart in the reference). */
+int nn_assembly_info(const nn_assembly *a, nn_context **ctx, int64_t *ncell,
+                     int64_t *nnzb, int32_t *np);
+
+/* ---- serving: continuous batching of Newton-step requests (no counterpart in
+ * the reference, which assembles one simulation per process) ----------------
+ *
+ * Ensemble members keep their state on the device (primary + phase-state
+ * vectors: each cell's K-values and phase hysteresis) and submit steps: the
+ * phase update (nn_phase_update_batch) then residual + BSR Jacobian
+ * (nn_assembly_evaluate) of the member's grid.  A worker thread admits pending
+ * steps in submission order as soon as a stream slot is free, the member has no
+ * step in flight, and the step's output buffers fit the memory budget; admitted
+ * steps run concurrently on the server's streams (one per slot).  Steps of one
+ * member run one at a time in submission order, so results do not depend on the
+ * mix.  Each member needs its own context: a context's work queue serves
+ * one launch at a time.  Output buffers belong to the step until
+ * nn_server_release.  max_inflight = 1 gives the plain one-request-at-a-time form.
+ */
+typedef struct nn_server nn_server;
+typedef struct nn_step_request {
+    nn_assembly *assembly;       /* member's grid; its context runs the phase update */
+    int64_t member;              /* member id (serialises that member's steps)       */
+    int32_t has_well_unknown;
+    int32_t want_bsr;            /* 0: residual only                                 */
+    double dt, fugacity_scaling;
+    const double *d_T;           /* per-cell temperature                             */
+    double *d_primary;           /* member state, updated in place by the phase update */
+    double *d_phase_state;
+    const double *d_accum_prev;  /* accepted-step accumulation                       */
+} nn_step_request;
+typedef struct nn_step_result {
+    int32_t rc;                  /* NN_OK or the failing call's code                 */
+    int32_t np;
+    int64_t ncell, nnzb;
+    double submit_s, start_s, end_s;  /* server clock: submission, launch, completion seen */
+    double *d_residual;          /* ncell x np                                       */
+    double *d_bsr;               /* nnzb x np x np (NULL without want_bsr)           */
+    int32_t *d_update_status;    /* ncell NN_STATUS_* of the phase update            */
+    int32_t *d_assembly_status;  /* ncell                                            */
+    uint8_t *d_update_flags;     /* ncell                                            */
+} nn_step_result;
+typedef struct nn_server_stats {
+    int64_t completed, pending;
+    int64_t inflight, peak_inflight;
+    int64_t memory_in_use, peak_memory_in_use, cached_bytes;
+} nn_server_stats;
+int nn_server_create(int32_t max_inflight, int64_t memory_budget_bytes, nn_server **out);
+int nn_server_destroy(nn_server *s);   /* finishes every submitted step first */
+int nn_server_submit_step(nn_server *s, const nn_step_request *req, int64_t *ticket);
+int nn_server_wait(nn_server *s, int64_t ticket, nn_step_result *res);  /* blocks */
+int nn_server_release(nn_server *s, int64_t ticket);
+int nn_server_get_stats(nn_server *s, nn_server_stats *out);
+const char *nn_server_last_error(void);
+
+/* ---- learned phase-count / log-K initialiser (no counterpart in the reference) ----
+ *
+ * MLP: features (ln P - 15.5)/1.5, (T - 450)/150, z[0..nc-1] -> three hidden
+ * layers of 64 SiLU units -> 3 phase-count logits (1, 2, 3 phases) + nc values of
+ * ln K_go.  FP32 training with the library's own forward/backward kernels and
+ * Adam (beta 0.9/0.999, eps 1e-8); BF16 inference (weights and activations
+ * rounded to BF16, FP32 accumulation).  Parameters: per layer W (out x in,
+ * row-major) then b (out), layers in order, FP32 on the device.
+ * Loss per state, averaged over the batch:
+ *   -ln softmax(o[0..2])[label] + w * mask * (1/nc) sum_k (o[3+k] - logk[k])^2.
+ */
+#define NN_MLP_MAX_BATCH (1 << 20)
+typedef struct nn_mlp nn_mlp;
+int nn_mlp_create(int32_t nc, uint64_t seed, nn_mlp **out);   /* nc in {3, 4, 6} */
+int nn_mlp_destroy(nn_mlp *m);
+int64_t nn_mlp_param_count(const nn_mlp *m);
+float *nn_mlp_params(nn_mlp *m);                 /* device pointer, param_count floats */
+int nn_mlp_reset_optimizer(nn_mlp *m);            /* zero Adam moments and step count */
+/* copy the parameters out to / in from a device array of param_count floats */
+int nn_mlp_get_params(nn_mlp *m, float *d_out, void *stream);
+int nn_mlp_set_params(nn_mlp *m, const float *d_in, void *stream);
+const char *nn_mlp_last_error(void);
+/* d_X: n x (nc+2) FP32 features of flash inputs */
+int nn_mlp_features(nn_mlp *m, int64_t n, const double *d_P, const double *d_T,
+                    const double *d_z, float *d_X, void *stream);
+/* Loss (d_loss[0]) and its gradient w.r.t. the parameters (d_grad), no update.
+ * d_label: class 0..2 (= phase count - 1); d_logk n x nc targets; d_mask: 1 where
+ * the log-K target applies.  n <= NN_MLP_MAX_BATCH. */
+int nn_mlp_loss_grad(nn_mlp *m, int64_t n, const float *d_X, const int32_t *d_label,
+                     const float *d_logk, const uint8_t *d_mask, float logk_weight,
+                     float *d_loss, float *d_grad, void *stream);
+/* One Adam step on a mini-batch (loss before the step in d_loss[0]). */
+int nn_mlp_train_step(nn_mlp *m, int64_t n, const float *d_X, const int32_t *d_label,
+                      const float *d_logk, const uint8_t *d_mask, float logk_weight,
+                      float lr, float *d_loss, void *stream);
+/* BF16 inference: d_prob n x 3 class probabilities, d_logk n x nc. */
+int nn_mlp_predict(nn_mlp *m, int64_t n, const float *d_X, float *d_prob, float *d_logk,
+                   void *stream);
+/* Flash hints from predictions: NN_HINT_SINGLE_PHASE where p(1 phase) >= threshold
+ * and modes has that bit, else NN_HINT_LOG_K where p(2) or p(3) >= threshold and
+ * modes has that bit, else NN_HINT_NONE; d_seed_log_k (n x nc FP64) = predicted log-K. */
+int nn_mlp_hints(nn_mlp *m, int64_t n, const float *d_prob, const float *d_logk,
+                 float threshold, int32_t modes, uint8_t *d_hint, double *d_seed_log_k,
+                 void *stream);
 
 #ifdef __cplusplus
 }

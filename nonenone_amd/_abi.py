@@ -115,6 +115,26 @@ class FlashOut(ctypes.Structure):
     ]
 
 
+class Schedule(ctypes.Structure):
+    """nn_schedule: launch shapes of the flash-like batches of a context."""
+
+    _fields_ = [
+        ("persistent_blocks_per_cu", _I32),
+        ("coop_blocks_per_cu", _I32),
+        ("ordered", _I32),
+        ("wave_claim", _I32),
+        ("plain", _I32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+NN_HINT_NONE = 0
+NN_HINT_SINGLE_PHASE = 1
+NN_HINT_LOG_K = 2
+NN_MLP_MAX_BATCH = 1 << 20
+
 _PRODUCT_LIB = None
 
 
@@ -160,10 +180,40 @@ def _declare(L: ctypes.CDLL) -> None:
     L.nn_assembly_evaluate.argtypes = [_P, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P]
     L.nn_context_set_kernel_timing.argtypes = [_P, _I32]
     L.nn_context_kernel_time.argtypes = [_P, _I32, ctypes.POINTER(_D), ctypes.POINTER(_I64)]
+    L.nn_flash_seeded_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P, ctypes.POINTER(FlashOut),
+                                        _P]
+    L.nn_context_get_schedule.argtypes = [_P, ctypes.POINTER(Schedule)]
+    L.nn_context_set_schedule.argtypes = [_P, ctypes.POINTER(Schedule)]
+    L.nn_flash_autotune.argtypes = [_P, _I64, _P, _P, _P, ctypes.POINTER(FlashOut), _I32,
+                                    ctypes.POINTER(Schedule), ctypes.POINTER(_D), _P]
+    L.nn_mlp_create.argtypes = [_I32, ctypes.c_uint64, ctypes.POINTER(_P)]
+    L.nn_mlp_destroy.argtypes = [_P]
+    L.nn_mlp_param_count.argtypes = [_P]
+    L.nn_mlp_param_count.restype = _I64
+    L.nn_mlp_params.argtypes = [_P]
+    L.nn_mlp_params.restype = _P
+    L.nn_mlp_reset_optimizer.argtypes = [_P]
+    L.nn_mlp_get_params.argtypes = [_P, _P, _P]
+    L.nn_mlp_set_params.argtypes = [_P, _P, _P]
+    L.nn_mlp_last_error.restype = ctypes.c_char_p
+    L.nn_mlp_features.argtypes = [_P, _I64, _P, _P, _P, _P, _P]
+    L.nn_mlp_loss_grad.argtypes = [_P, _I64, _P, _P, _P, _P, ctypes.c_float, _P, _P, _P]
+    L.nn_mlp_train_step.argtypes = [_P, _I64, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, _P,
+                                    _P]
+    L.nn_mlp_predict.argtypes = [_P, _I64, _P, _P, _P, _P]
+    L.nn_mlp_hints.argtypes = [_P, _I64, _P, _P, ctypes.c_float, _I32, _P, _P, _P]
+    L.nn_assembly_info.argtypes = [_P, ctypes.POINTER(_P), ctypes.POINTER(_I64),
+                                   ctypes.POINTER(_I64), ctypes.POINTER(_I32)]
+    L.nn_assembly_info.restype = ctypes.c_int
     for name in ("nn_context_set_kernel_timing", "nn_context_kernel_time","nn_context_create", "nn_context_destroy", "nn_eos_phase_batch",
                  "nn_flash_batch", "nn_flash_restricted_batch", "nn_stability_batch",
                  "nn_phase_update_batch", "nn_structured_topology", "nn_assembly_create",
-                 "nn_assembly_destroy", "nn_assembly_accumulation", "nn_assembly_evaluate"):
+                 "nn_assembly_destroy", "nn_assembly_accumulation", "nn_assembly_evaluate",
+                 "nn_flash_seeded_batch", "nn_context_get_schedule", "nn_context_set_schedule",
+                 "nn_flash_autotune", "nn_mlp_create", "nn_mlp_destroy",
+                 "nn_mlp_reset_optimizer", "nn_mlp_features", "nn_mlp_loss_grad",
+                 "nn_mlp_train_step", "nn_mlp_predict", "nn_mlp_hints", "nn_mlp_get_params",
+                 "nn_mlp_set_params"):
         getattr(L, name).restype = ctypes.c_int
 
 
@@ -174,6 +224,13 @@ EXPORTED_SYMBOLS = (
     "nn_stability_batch", "nn_phase_update_batch", "nn_structured_topology",
     "nn_assembly_create", "nn_assembly_destroy", "nn_assembly_accumulation",
     "nn_assembly_evaluate", "nn_context_set_kernel_timing", "nn_context_kernel_time",
+    "nn_flash_seeded_batch", "nn_context_get_schedule", "nn_context_set_schedule",
+    "nn_flash_autotune", "nn_mlp_create", "nn_mlp_destroy", "nn_mlp_param_count",
+    "nn_mlp_params", "nn_mlp_reset_optimizer", "nn_mlp_last_error", "nn_mlp_features",
+    "nn_mlp_loss_grad", "nn_mlp_train_step", "nn_mlp_predict", "nn_mlp_hints",
+    "nn_mlp_get_params", "nn_mlp_set_params", "nn_assembly_info",
+    "nn_server_create", "nn_server_destroy", "nn_server_submit_step", "nn_server_wait",
+    "nn_server_release", "nn_server_get_stats", "nn_server_last_error",
 )
 
 # Kernel kinds of nn_context_kernel_time.
@@ -183,5 +240,6 @@ KERNEL_KINDS = {"eos": 0, "flash": 1, "stability": 2, "phase_update": 3, "cell_r
 
 def check(rc: int, what: str) -> None:
     if rc != NN_OK:
-        msg = lib().nn_last_error()
+        getter = lib().nn_mlp_last_error if what.startswith("nn_mlp") else lib().nn_last_error
+        msg = getter()
         raise RuntimeError(f"{what} failed with code {rc}: {msg.decode() if msg else ''}")

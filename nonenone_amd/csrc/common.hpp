@@ -37,6 +37,40 @@ enum : int
     // Internal: the state exceeded the lane-per-state work budget and is handed
     // to the wave-cooperative pass; never reported to callers.
     kErrDefer = 64,
+    // Internal: the phase update recorded its stability test's seed-trial inputs
+    // for the lockstep trial pass and stops; never reported to callers.
+    kErrPending = 65,
+};
+
+// Stability-test record of one cell (phase update, three passes): the
+// lane-per-cell pass records the inputs of the seed trials, the lockstep trial
+// pass fills in the best trial of every missing candidate phase, and the
+// finishing pass replays the stability test from them.
+enum : int
+{
+    kStabNone = 0,
+    kStabRecord = 1,
+    kStabReplay = 2,
+};
+
+template <int NC>
+struct StabRecord
+{
+    double P, T;
+    double z[NC], refComp[NC], logRef[NC];
+    int32_t active;
+    int32_t errCode;  // error a trial raised: the replay raises it where the serial test stops
+    double sum[3];
+    double comp[3][NC];
+    uint8_t valid[3], trivial[3];
+};
+
+// Outcome of one item of a lane-per-item pass.
+enum : int
+{
+    kItemDone = 0,
+    kItemDeferred = 1,
+    kItemPending = 2,
 };
 
 // Execution modes of the flash code.

@@ -171,6 +171,17 @@ public:
     bool nearTie = false;
     // RR3 residual evaluations so far (kModeDefer's budget; per lane in kModeCoop).
     mutable int work = 0;
+    // Learned-initialiser hint for this solver's first gas-oil start (no
+    // counterpart in the reference): NN_HINT_SINGLE_PHASE skips that start and
+    // goes to the single-phase candidate, NN_HINT_LOG_K replaces its Wilson K by
+    // exp(hintLogK).  The active-set equilibration's stability tests judge the
+    // outcome either way; solvers made for recovery paths carry no hint.
+    int hint = 0;
+    const double *hintLogK = nullptr;
+    // Record/replay of the next stabilityTest's seed trials (kStabRecord /
+    // kStabReplay, see StabRecord); reset to kStabNone by that call.
+    int stabMode = kStabNone;
+    StabRecord<NC> *stabRec = nullptr;
 
     __host__ __device__ Flash(const FluidParams &fp, const FlashOptions &op, Err &e)
         : f(fp), o(op), err(e), enablePermutation(true), enableContinuation(true)
@@ -1868,6 +1879,254 @@ public:
         return false;
     }
 
+    // ------------------------------------------- lockstep seed trials
+    // Every seed trial of a recorded stability test (all missing candidates,
+    // all seeds, in the serial order) as a per-lane state machine.  One step of
+    // the machine is one residual evaluation at the point trialPoint() names,
+    // so a wavefront's lanes evaluate the EOS together whatever trial, seed or
+    // stage each lane is in; trialAdvance() then moves the lane on exactly as
+    // stabilityTrial's loops would (same operations, same order, hence the same
+    // Trial per seed and the same best per candidate).
+    enum : int
+    {
+        kTrialSsi = 0,
+        kTrialNewtonBase = 1,
+        kTrialNewtonFd = 2,
+        kTrialLineSearch = 3,
+        kTrialSsiTail = 4,
+    };
+
+    struct TrialMachine
+    {
+        int cand, k, stage, it, col, ls;
+        double damping;
+        double logW[NC], delta[NC];
+        double jac[NC][NC];
+        StabResidual cur;
+        Trial best;
+    };
+
+    __host__ __device__ int trialSsiSteps() const
+    {
+        return o.maxStabilityIterations < (o.stabilitySsi > 0 ? o.stabilitySsi : 0)
+                   ? o.maxStabilityIterations
+                   : (o.stabilitySsi > 0 ? o.stabilitySsi : 0);
+    }
+
+    // Next stage after the SSI stage (or after Newton stops): returns false when
+    // the trial ends there without another evaluation (an invalid Trial).
+    __host__ __device__ bool trialEnterNewton(TrialMachine &m) const
+    {
+        if (o.maxStabilityNewton > 0)
+        {
+            m.stage = kTrialNewtonBase;
+            m.it = 0;
+            return true;
+        }
+        return trialEnterTail(m);
+    }
+
+    __host__ __device__ bool trialEnterTail(TrialMachine &m) const
+    {
+        const int remaining = o.maxStabilityIterations - trialSsiSteps() > 0
+                                  ? o.maxStabilityIterations - trialSsiSteps()
+                                  : 0;
+        m.stage = kTrialSsiTail;
+        m.it = 0;
+        return remaining > 0;
+    }
+
+    // Starts seed m.k of candidate m.cand; false when no such seed (or the trial
+    // ends before its first evaluation, recorded through `ended`).
+    __host__ __device__ bool trialBeginSeed(const StabRecord<NC> &r, TrialMachine &m, bool &ended)
+    {
+        ended = false;
+        double seed[NC];
+        if (!stabilitySeed(m.k, r.P, r.T, r.z, r.refComp, seed))
+            return false;
+        if (err.bad())
+            return true;
+        normalize(seed);
+        for (int c = 0; c < NC; ++c) m.logW[c] = ::log(maxd(seed[c], o.floor));
+        bool live;
+        if (trialSsiSteps() > 0)
+        {
+            m.stage = kTrialSsi;
+            m.it = 0;
+            live = true;
+        }
+        else
+        {
+            live = trialEnterNewton(m);
+        }
+        ended = !live;
+        return true;
+    }
+
+    __host__ __device__ void trialPoint(const TrialMachine &m, double *x) const
+    {
+        for (int c = 0; c < NC; ++c) x[c] = m.logW[c];
+        if (m.stage == kTrialNewtonFd)
+            x[m.col] += o.fdStep * maxd(1.0, ::fabs(m.logW[m.col]));
+        else if (m.stage == kTrialLineSearch)
+            for (int c = 0; c < NC; ++c) x[c] = m.logW[c] + m.damping * m.delta[c];
+    }
+
+    // Applies the residual at trialPoint(m); returns true when the current
+    // trial ended, with its outcome in `t`.
+    __host__ __device__ bool trialStep(const StabRecord<NC> &r, TrialMachine &m,
+                                       const StabResidual &s, Trial &t)
+    {
+        t = Trial();
+        switch (m.stage)
+        {
+        case kTrialSsi:
+        case kTrialSsiTail:
+        {
+            if (err.bad() || !s.valid)
+                return true;
+            if (s.norm < o.stabTol)
+            {
+                t = finalizeTrial(s, r.refComp);
+                return true;
+            }
+            for (int c = 0; c < NC; ++c)
+                m.logW[c] = r.logRef[c] - ::log(maxd(s.phi[c], 1.0e-300));
+            ++m.it;
+            if (m.stage == kTrialSsi)
+            {
+                if (m.it >= trialSsiSteps())
+                    return !trialEnterNewton(m);
+                return false;
+            }
+            const int remaining = o.maxStabilityIterations - trialSsiSteps();
+            return m.it >= remaining;
+        }
+        case kTrialNewtonBase:
+            if (err.bad() || !s.valid)
+                return true;
+            if (s.norm < o.stabTol)
+            {
+                t = finalizeTrial(s, r.refComp);
+                return true;
+            }
+            m.cur = s;
+            m.col = 0;
+            m.stage = kTrialNewtonFd;
+            return false;
+        case kTrialNewtonFd:
+        {
+            if (err.bad() || !s.valid)
+                return true;
+            const double step = o.fdStep * maxd(1.0, ::fabs(m.logW[m.col]));
+            for (int row = 0; row < NC; ++row)
+                m.jac[row][m.col] = (s.residual[row] - m.cur.residual[row]) / step;
+            if (++m.col < NC)
+                return false;
+            double rhs[NC];
+            for (int c = 0; c < NC; ++c) rhs[c] = -m.cur.residual[c];
+            if (!solveDense<NC>(m.jac, rhs, m.delta))
+                return !trialEnterTail(m);
+            m.ls = 0;
+            m.damping = 1.0;
+            m.stage = kTrialLineSearch;
+            return false;
+        }
+        default:  // kTrialLineSearch
+            if (err.bad())
+                return true;
+            if (s.valid && s.norm < m.cur.norm)
+            {
+                for (int c = 0; c < NC; ++c) m.logW[c] = m.logW[c] + m.damping * m.delta[c];
+                if (++m.it >= o.maxStabilityNewton)
+                    return !trialEnterTail(m);
+                m.stage = kTrialNewtonBase;
+                return false;
+            }
+            m.damping *= 0.5;
+            if (++m.ls >= 16)
+                return !trialEnterTail(m);
+            return false;
+        }
+    }
+
+    // First missing candidate at or after `from` (3 when none).
+    __host__ __device__ static int nextMissing(uint8_t active, int from)
+    {
+        int c = from;
+        while (c < 3 && (active & (1u << c)))
+            ++c;
+        return c;
+    }
+
+    // Moves the machine to its next seed (or candidate) after a trial ended;
+    // returns true when the record is complete (or an error ended it).
+    __host__ __device__ bool trialNext(StabRecord<NC> &r, TrialMachine &m)
+    {
+        for (;;)
+        {
+            if (err.bad())
+            {
+                r.errCode = err.code;
+                return true;
+            }
+            ++m.k;
+            bool ended = false;
+            if (trialBeginSeed(r, m, ended))
+            {
+                if (err.bad())
+                    continue;
+                if (!ended)
+                    return false;
+                // a trial that ends before any evaluation is invalid: keep best
+                continue;
+            }
+            // candidate finished
+            r.valid[m.cand] = m.best.valid ? 1 : 0;
+            r.trivial[m.cand] = m.best.trivial ? 1 : 0;
+            r.sum[m.cand] = m.best.sum;
+            for (int c = 0; c < NC; ++c) r.comp[m.cand][c] = m.best.comp[c];
+            m.cand = nextMissing(uint8_t(r.active), m.cand + 1);
+            if (m.cand >= 3)
+                return true;
+            m.best = Trial();
+            m.best.sum = -__builtin_inf();
+            m.k = -1;
+        }
+    }
+
+    // Initialises the machine for a recorded cell; true when it is already complete.
+    __host__ __device__ bool trialInit(StabRecord<NC> &r, TrialMachine &m)
+    {
+        r.errCode = kErrNone;
+        for (int p = 0; p < 3; ++p)
+        {
+            r.valid[p] = 0;
+            r.trivial[p] = 0;
+            r.sum[p] = -__builtin_inf();
+        }
+        m.cand = nextMissing(uint8_t(r.active), 0);
+        if (m.cand >= 3)
+            return true;
+        m.best = Trial();
+        m.best.sum = -__builtin_inf();
+        m.k = -1;
+        return trialNext(r, m);
+    }
+
+    // After trialStep reported the end of a trial: fold it into the candidate's best.
+    __host__ __device__ bool trialFold(StabRecord<NC> &r, TrialMachine &m, const Trial &t)
+    {
+        if (err.bad())
+        {
+            r.errCode = err.code;
+            return true;
+        }
+        if (t.valid && (!m.best.valid || t.sum > m.best.sum))
+            m.best = t;
+        return trialNext(r, m);
+    }
+
     __host__ __device__ Result flashRestrictedImpl(double P, double T, const double *zin,
                                                    uint8_t allowed, const double (*seed)[NC]);
 
@@ -1899,6 +2158,9 @@ public:
         if (err.bad())
             return result;
 
+        const int stabUse = stabRec ? stabMode : kStabNone;
+        stabMode = kStabNone;
+        bool certifiedRun = false;
         if (sw() && popcount3(active) > 1)
         {
             PhaseThermo<NC> rawRef;
@@ -1926,6 +2188,7 @@ public:
             const double certTol = maxd(10.0 * o.fugTol, 10.0 * o.stabTol);
             if (mismatch > certTol)
             {
+                certifiedRun = true;
                 Result certified = flashRestrictedImpl(P, T, z, active, comps);
                 if (err.bad())
                     return result;
@@ -1956,14 +2219,46 @@ public:
             NN_TRACE("stab ref=%d c=%d x=%.17g phi=%.17g logRef=%.17g\n", ref, c, refComp[c], refThermo.phi[c], logRef[c]);
         }
 
+        if (stabUse == kStabRecord && !certifiedRun)
+        {
+            // Hand the seed trials to the lockstep pass (a state whose SW
+            // certification ran a restricted flash finishes here instead).
+            StabRecord<NC> &rec = *stabRec;
+            rec.P = P;
+            rec.T = T;
+            rec.active = active;
+            rec.errCode = kErrNone;
+            for (int c = 0; c < NC; ++c)
+            {
+                rec.z[c] = z[c];
+                rec.refComp[c] = refComp[c];
+                rec.logRef[c] = logRef[c];
+            }
+            err.raise(kErrPending);
+            return result;
+        }
+        const bool replay = stabUse == kStabReplay;
+        if (replay && stabRec->errCode != kErrNone)
+        {
+            err.raise(stabRec->errCode);
+            return result;
+        }
+
         for (int cand = 0; cand < 3; ++cand)
         {
             if (active & (1u << cand))
                 continue;
             Trial best;
             best.sum = -__builtin_inf();
+            if (replay)
+            {
+                best.valid = stabRec->valid[cand] != 0;
+                best.trivial = stabRec->trivial[cand] != 0;
+                best.sum = stabRec->sum[cand];
+                for (int c = 0; c < NC; ++c) best.comp[c] = stabRec->comp[cand][c];
+            }
             double seed[NC];
-            for (int k = 0; stabilitySeed(k, P, T, z, refComp, seed); ++k)
+            for (int k = 0; !replay && stabilitySeed(k, P, T, z, refComp, seed); ++k)
             {
                 if (err.bad())
                     return result;
@@ -2986,7 +3281,15 @@ public:
         wilsonK(P, T, kg);
         initialWaterOilK(kw);
 
-        Result seed = flashTwo(P, T, z, kOil, kGas, kg);
+        Result seed;
+        if (hint != NN_HINT_SINGLE_PHASE)
+        {
+            double k0[NC];
+            for (int c = 0; c < NC; ++c)
+                k0[c] = (hint == NN_HINT_LOG_K && hintLogK) ? ::exp(clampd(hintLogK[c], -230.0, 230.0))
+                                                           : kg[c];
+            seed = flashTwo(P, T, z, kOil, kGas, k0);
+        }
         NN_TRACE("seed conv=%d it=%d beta=%.17g %.17g %.17g\n", (int)seed.converged, seed.iterations, seed.beta[0], seed.beta[1], seed.beta[2]);
         if (err.bad())
             return seed;

@@ -84,7 +84,8 @@ __device__ void storeFlash(const FlashResult<NC> &r, int64_t s, const nn_flash_o
 template <int NC, int Mode>
 __device__ bool flashOne(const FluidParams &f, const FlashOptions &o, int64_t s,
                          const double *P, const double *T, const double *z,
-                         const uint8_t *allowed, const double *seed, const nn_flash_out &out)
+                         const uint8_t *allowed, const double *seed, FlashHint h,
+                         const nn_flash_out &out)
 {
     Err err;
     Flash<NC, Mode> fl(f, o, err);
@@ -93,6 +94,11 @@ __device__ bool flashOne(const FluidParams &f, const FlashOptions &o, int64_t s,
     FlashResult<NC> r;
     if (allowed == nullptr)
     {
+        if (h.hint)
+        {
+            fl.hint = h.hint[s];
+            fl.hintLogK = h.logK ? h.logK + s * NC : nullptr;
+        }
         r = fl.flash(P[s], T[s], zs);
     }
     else
@@ -160,25 +166,78 @@ __device__ inline int64_t nextDeferred(WorkQueue *q, const unsigned int *list)
     return static_cast<int64_t>(list[idx]);
 }
 
+// Drains the lane-per-item queue of a persistent pass, calling body(i) for
+// each claimed position i < n.  q->claim == 0: every lane claims one item at a
+// time, so a lane that finishes takes new work at once; q->claim != 0: lane 0
+// claims a tile of 64 consecutive positions for its wavefront, so the lanes of a
+// wavefront stay on neighbouring items of the processing order (less divergence
+// when the order is sorted by condition).
+template <class F>
+__device__ inline void drainCounter(WorkQueue *q, unsigned int *head, int64_t n, F &&body)
+{
+    if (q->claim)
+    {
+        for (;;)
+        {
+            unsigned int base = 0;
+            if (waveLane() == 0)
+                base = atomicAdd(head, unsigned(kBlock));
+            base = static_cast<unsigned int>(waveShfl(static_cast<int>(base), 0));
+            if (int64_t(base) >= n)
+                break;
+            const int64_t i = int64_t(base) + waveLane();
+            if (i < n)
+                body(i);
+        }
+    }
+    else
+    {
+        for (;;)
+        {
+            const int64_t i = int64_t(atomicAdd(head, 1u));
+            if (i >= n)
+                break;
+            body(i);
+        }
+    }
+}
+
+template <class F>
+__device__ inline void drainQueue(WorkQueue *q, int64_t n, F &&body)
+{
+    drainCounter(q, &q->head, n, static_cast<F &&>(body));
+}
+
 template <int NC>
 __global__ void __launch_bounds__(kBlock) flashKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
-    const uint8_t *__restrict__ allowed, const double *__restrict__ seed, nn_flash_out out,
-    WorkQueue *__restrict__ q, unsigned int *__restrict__ list,
+    const uint8_t *__restrict__ allowed, const double *__restrict__ seed, FlashHint hint,
+    nn_flash_out out, WorkQueue *__restrict__ q, unsigned int *__restrict__ list,
     const unsigned int *__restrict__ order)
 {
-    // Persistent work queue: each work-item claims states until none are left,
-    // in `order` (states of similar conditions next to each other) when given.
-    for (;;)
-    {
-        const int64_t i = int64_t(atomicAdd(&q->head, 1u));
-        if (i >= n)
-            break;
+    // Persistent work queue: work-items claim states until none are left, in
+    // `order` (states of similar conditions next to each other) when given.
+    drainQueue(q, n, [&](int64_t i) {
         const int64_t s = order ? int64_t(order[i]) : i;
-        if (q->deferAll || !flashOne<NC, kModeDefer>(*fp, *op, s, P, T, z, allowed, seed, out))
+        if (q->deferAll ||
+            !flashOne<NC, kModeDefer>(*fp, *op, s, P, T, z, allowed, seed, hint, out))
             deferItem(q, list, s);
-    }
+    });
+}
+
+// The baseline form: one work-item per state in index order, no queue, no deferral.
+template <int NC>
+__global__ void __launch_bounds__(kBlock) flashPlainKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
+    const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
+    const uint8_t *__restrict__ allowed, const double *__restrict__ seed, FlashHint hint,
+    nn_flash_out out)
+{
+    const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (s >= n)
+        return;
+    flashOne<NC, kModePlain>(*fp, *op, s, P, T, z, allowed, seed, hint, out);
 }
 
 // Sort keys that put states of similar (T, P, z) next to each other, so that
@@ -194,7 +253,8 @@ __device__ inline unsigned int quantize(double v, double lo, double hi, int bits
 template <int NC>
 __global__ void flashKeyKernel(int64_t n, const double *__restrict__ P,
                                const double *__restrict__ T, const double *__restrict__ z,
-                               unsigned int *__restrict__ keys, unsigned int *__restrict__ idx)
+                               const uint8_t *__restrict__ hint, unsigned int *__restrict__ keys,
+                               unsigned int *__restrict__ idx)
 {
     const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (s >= n)
@@ -204,6 +264,8 @@ __global__ void flashKeyKernel(int64_t n, const double *__restrict__ P,
     k |= quantize(lp, 11.5, 18.5, 10) << 12;
     k |= quantize(z[s * NC], 0.0, 1.0, 6) << 6;
     k |= quantize(z[s * NC + 1], 0.0, 1.0, 6);
+    if (hint)  // learned hint first: states on the same solver path share wavefronts
+        k = (unsigned int)(hint[s] & 3u) << 30 | (k >> 2);
     keys[s] = k;
     idx[s] = static_cast<unsigned int>(s);
 }
@@ -213,15 +275,15 @@ template <int NC>
 __global__ void __launch_bounds__(kBlock) flashCoopKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
-    const uint8_t *__restrict__ allowed, const double *__restrict__ seed, nn_flash_out out,
-    WorkQueue *__restrict__ q, const unsigned int *__restrict__ list)
+    const uint8_t *__restrict__ allowed, const double *__restrict__ seed, FlashHint hint,
+    nn_flash_out out, WorkQueue *__restrict__ q, const unsigned int *__restrict__ list)
 {
     for (;;)
     {
         const int64_t s = nextDeferred(q, list);
         if (s < 0)
             break;
-        flashOne<NC, kModeCoop>(*fp, *op, s, P, T, z, allowed, seed, out);
+        flashOne<NC, kModeCoop>(*fp, *op, s, P, T, z, allowed, seed, hint, out);
     }
 }
 
@@ -261,18 +323,83 @@ __global__ void __launch_bounds__(kBlock) phaseUpdateKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
     int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
-    unsigned int *__restrict__ list, const unsigned int *__restrict__ order)
+    unsigned int *__restrict__ list, const unsigned int *__restrict__ order,
+    unsigned int *__restrict__ pendingList, StabRecord<NC> *__restrict__ records)
 {
+    drainQueue(q, n, [&](int64_t i) {
+        const int64_t s = order ? int64_t(order[i]) : i;
+        if (q->deferAll)
+        {
+            deferItem(q, list, s);
+            return;
+        }
+        const int r = phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary, phaseState, s,
+                                                            status, flags, kStabRecord, records);
+        if (r == kItemDeferred)
+            deferItem(q, list, s);
+        else if (r == kItemPending)
+            pendingList[atomicAdd(&q->pending, 1u)] = static_cast<unsigned int>(s);
+    });
+}
+
+// Lockstep stability trials of the recorded cells: each lane runs every seed
+// trial of its cell as a state machine (Flash::trialStep) whose one expensive
+// step, the residual evaluation, all lanes of the wavefront take together; a
+// lane whose cell is complete claims the next recorded cell.
+template <int NC>
+__global__ void __launch_bounds__(kBlock) stabilityTrialKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
+    WorkQueue *__restrict__ q, const unsigned int *__restrict__ pendingList,
+    StabRecord<NC> *__restrict__ records)
+{
+    using F = Flash<NC, kModePlain>;
+    const unsigned int count = q->pending;
+    Err err;
+    F fl(*fp, *op, err);
+    typename F::TrialMachine m;
+    StabRecord<NC> *rec = nullptr;
     for (;;)
     {
-        const int64_t i = int64_t(atomicAdd(&q->head, 1u));
-        if (i >= n)
+        while (rec == nullptr)
+        {
+            const unsigned int slot = atomicAdd(&q->trialHead, 1u);
+            if (slot >= count)
+                break;
+            rec = records + pendingList[slot];
+            err.code = kErrNone;
+            if (fl.trialInit(*rec, m))
+                rec = nullptr;  // nothing to evaluate (or an error): complete already
+        }
+        if (rec == nullptr)
             break;
-        const int64_t s = order ? int64_t(order[i]) : i;
-        if (q->deferAll || !phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary,
-                                                                  phaseState, s, status, flags))
-            deferItem(q, list, s);
+        double x[NC];
+        fl.trialPoint(m, x);
+        typename F::StabResidual res;
+        fl.evaluateStabilityResidual(m.cand, rec->P, rec->T, rec->logRef, x, res);
+        typename F::Trial t;
+        if (fl.trialStep(*rec, m, res, t) && fl.trialFold(*rec, m, t))
+            rec = nullptr;
     }
+}
+
+// Finishing pass of the recorded cells: the phase update again, with the
+// stability test replayed from the lockstep trials' results.
+template <int NC, bool Well>
+__global__ void __launch_bounds__(kBlock) phaseUpdateReplayKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
+    const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
+    int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
+    unsigned int *__restrict__ list, const unsigned int *__restrict__ pendingList,
+    StabRecord<NC> *__restrict__ records)
+{
+    const int64_t count = int64_t(q->pending);
+    drainCounter(q, &q->replayHead, count, [&](int64_t i) {
+        const int64_t s = int64_t(pendingList[i]);
+        const int r = phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary, phaseState, s,
+                                                            status, flags, kStabReplay, records);
+        if (r == kItemDeferred)
+            deferItem(q, list, s);
+    });
 }
 
 // Phase-update sort key: presence, whether an active saturation sits at or
@@ -316,7 +443,7 @@ __global__ void __launch_bounds__(kBlock) phaseUpdateCoopKernel(
         const int64_t s = nextDeferred(q, list);
         if (s < 0)
             break;
-        phaseUpdateCell<NC, Well, kModeCoop>(*fp, *op, T[s], primary, phaseState, s, status,
+        (void)phaseUpdateCell<NC, Well, kModeCoop>(*fp, *op, T[s], primary, phaseState, s, status,
                                              flags);
     }
 }
@@ -533,12 +660,19 @@ template <int NC>
 hipError_t Launch<NC>::flash(unsigned grid, unsigned coopGrid, hipStream_t st,
                              const FluidParams *fp, const FlashOptions *op, int64_t n,
                              const double *P, const double *T, const double *z,
-                             const uint8_t *allowed, const double *seed, nn_flash_out out,
-                             WorkQueue *q, unsigned int *list, const unsigned int *order,
-                             hipEvent_t splitA, hipEvent_t splitB)
+                             const uint8_t *allowed, const double *seed, FlashHint hint,
+                             nn_flash_out out, WorkQueue *q, unsigned int *list,
+                             const unsigned int *order, bool plain, hipEvent_t splitA,
+                             hipEvent_t splitB)
 {
-    flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, out, q, list,
-                                             order);
+    if (plain)
+    {
+        flashPlainKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, hint,
+                                                      out);
+        return hipGetLastError();
+    }
+    flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, hint, out, q,
+                                             list, order);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
@@ -546,16 +680,18 @@ hipError_t Launch<NC>::flash(unsigned grid, unsigned coopGrid, hipStream_t st,
         return e;
     if (splitB && (e = hipEventRecord(splitB, st)) != hipSuccess)
         return e;
-    flashCoopKernel<NC><<<coopGrid, kBlock, 0, st>>>(fp, op, P, T, z, allowed, seed, out, q, list);
+    flashCoopKernel<NC><<<coopGrid, kBlock, 0, st>>>(fp, op, P, T, z, allowed, seed, hint, out,
+                                                     q, list);
     return hipGetLastError();
 }
 
 template <int NC>
 hipError_t Launch<NC>::flashKeys(hipStream_t st, int64_t n, const double *P, const double *T,
-                                 const double *z, unsigned int *keys, unsigned int *idx)
+                                 const double *z, const uint8_t *hint, unsigned int *keys,
+                                 unsigned int *idx)
 {
     const unsigned grid = static_cast<unsigned>((n + 255) / 256);
-    flashKeyKernel<NC><<<grid, 256, 0, st>>>(n, P, T, z, keys, idx);
+    flashKeyKernel<NC><<<grid, 256, 0, st>>>(n, P, T, z, hint, keys, idx);
     return hipGetLastError();
 }
 
@@ -591,17 +727,37 @@ hipError_t Launch<NC>::phaseUpdate(bool well, unsigned grid, unsigned coopGrid, 
                                    const double *T, double *primary, double *phaseState,
                                    int32_t *status, uint8_t *flags, WorkQueue *q,
                                    unsigned int *list, const unsigned int *order,
+                                   unsigned int *pendingList, void *records,
                                    hipEvent_t splitA, hipEvent_t splitB)
 {
+    StabRecord<NC> *recs = static_cast<StabRecord<NC> *>(records);
+    if (!pendingList)
+        recs = nullptr;
     if (well)
         phaseUpdateKernel<NC, true><<<grid, kBlock, 0, st>>>(fp, op, n, T, primary, phaseState,
-                                                             status, flags, q, list, order);
+                                                             status, flags, q, list, order,
+                                                             pendingList, recs);
     else
         phaseUpdateKernel<NC, false><<<grid, kBlock, 0, st>>>(fp, op, n, T, primary, phaseState,
-                                                              status, flags, q, list, order);
+                                                              status, flags, q, list, order,
+                                                              pendingList, recs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
+    if (recs)
+    {
+        stabilityTrialKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, q, pendingList, recs);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        if (well)
+            phaseUpdateReplayKernel<NC, true><<<grid, kBlock, 0, st>>>(
+                fp, op, T, primary, phaseState, status, flags, q, list, pendingList, recs);
+        else
+            phaseUpdateReplayKernel<NC, false><<<grid, kBlock, 0, st>>>(
+                fp, op, T, primary, phaseState, status, flags, q, list, pendingList, recs);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
     if (splitA && (e = hipEventRecord(splitA, st)) != hipSuccess)
         return e;
     if (splitB && (e = hipEventRecord(splitB, st)) != hipSuccess)

@@ -38,6 +38,18 @@ struct WorkQueue
     unsigned int deferred;  // items handed to the cooperative pass (= list length)
     unsigned int coopHead;  // next list entry of the cooperative pass
     unsigned int deferAll;  // nonzero: every item goes to the cooperative pass (testing)
+    unsigned int claim;     // 0: lanes claim single items; 1: wavefronts claim 64-item tiles
+    unsigned int pending;   // items recorded for the lockstep stability trials (list length)
+    unsigned int trialHead; // next pending item of the lockstep trial pass
+    unsigned int replayHead;// next pending item of the finishing pass
+};
+
+// Per-state hints of the learned initialiser (both may be null): hint[s] is
+// NN_HINT_*, logK is n x NC.
+struct FlashHint
+{
+    const uint8_t *hint;
+    const double *logK;
 };
 
 template <int NC>
@@ -48,26 +60,35 @@ struct Launch
                           double *Z, double *lnphi, double *fug, int32_t *status);
     // Lane-per-state pass, then one wavefront per deferred state.  `splitA`
     // and `splitB` (may be null) are recorded between the two launches.
+    // `plain`: one kernel, one work-item per state in index order, no deferral
+    // (the baseline form; `grid` is then ceil(n / kBlock)).
     static hipError_t flash(unsigned grid, unsigned coopGrid, hipStream_t st,
                             const FluidParams *fp, const FlashOptions *op, int64_t n,
                             const double *P, const double *T, const double *z,
-                            const uint8_t *allowed, const double *seed, nn_flash_out out,
-                            WorkQueue *q, unsigned int *list, const unsigned int *order,
-                            hipEvent_t splitA, hipEvent_t splitB);
+                            const uint8_t *allowed, const double *seed, FlashHint hint,
+                            nn_flash_out out, WorkQueue *q, unsigned int *list,
+                            const unsigned int *order, bool plain, hipEvent_t splitA,
+                            hipEvent_t splitB);
     static hipError_t stability(unsigned grid, hipStream_t st, const FluidParams *fp,
                                 const FlashOptions *op, int64_t n, const double *P,
                                 const double *T, const double *z, const uint8_t *active,
                                 const double *x, uint8_t *stable, uint8_t *valid, uint8_t *mask,
                                 double *trial, double *incipient, int32_t *status);
+    // `pendingList` / `records` (may be null: no lockstep trials): cells whose
+    // direct stability test is recorded by the lane-per-cell pass, run by the
+    // lockstep trial pass and finished by the replay pass, all before the
+    // cooperative pass.  `records` holds n StabRecord<NC>.
     static hipError_t phaseUpdate(bool well, unsigned grid, unsigned coopGrid, hipStream_t st,
                                   const FluidParams *fp, const FlashOptions *op, int64_t n,
                                   const double *T, double *primary, double *phaseState,
                                   int32_t *status, uint8_t *flags, WorkQueue *q,
                                   unsigned int *list, const unsigned int *order,
+                                  unsigned int *pendingList, void *records,
                                   hipEvent_t splitA, hipEvent_t splitB);
     // Sort keys (and identity indices) for the processing order of a batch.
     static hipError_t flashKeys(hipStream_t st, int64_t n, const double *P, const double *T,
-                                const double *z, unsigned int *keys, unsigned int *idx);
+                                const double *z, const uint8_t *hint, unsigned int *keys,
+                                unsigned int *idx);
     static hipError_t phaseUpdateKeys(bool well, hipStream_t st, int64_t n, const double *T,
                                       const double *primary, const double *phaseState,
                                       unsigned int *keys, unsigned int *idx);

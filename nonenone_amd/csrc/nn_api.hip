@@ -33,12 +33,21 @@ struct nn_context
     WorkQueue *dQueue = nullptr;       // counters of the two-pass persistent launches
     unsigned int *dList = nullptr;     // deferred item indices (capacity listCap)
     int64_t listCap = 0;
+    // Lockstep stability trials of the phase update (NN_STAB_LOCKSTEP=0 disables):
+    // recorded cell indices and one StabRecord per cell.
+    bool lockstep = true;
+    unsigned int *dPending = nullptr;
+    void *dRecords = nullptr;
+    int64_t recordCap = 0;
+    size_t recordBytes = 0;
     int numCUs = 256;
     int64_t maxBlocks = 0;  // NN_PERSISTENT_BLOCKS override of the persistent grid (0 = auto)
     int64_t coopBlocks = 0; // NN_COOP_BLOCKS override of the cooperative grid (0 = auto)
     bool coopAll = false;   // NN_COOP_ALL=1: every item through the cooperative pass (testing)
-    // Processing order of large batches (radix-sorted condition keys); NN_ORDER=0 disables.
-    bool ordering = true;
+    // Launch shapes of flash-like batches (nn_context_set_schedule / nn_flash_autotune).
+    // Processing order of large batches (radix-sorted condition keys): sched.ordered,
+    // NN_ORDER=0 disables.
+    nn_schedule sched{16, 4, 1, 0, 0};
     unsigned int *dKeys[2] = {nullptr, nullptr};
     unsigned int *dIdx[2] = {nullptr, nullptr};
     void *dSortTemp = nullptr;
@@ -81,20 +90,28 @@ int dispatchNc(int nc, F &&f)
     {
     case 3: return f(std::integral_constant<int, 3>{});
     case 4: return f(std::integral_constant<int, 4>{});
+    case 5: return f(std::integral_constant<int, 5>{});
     case 6: return f(std::integral_constant<int, 6>{});
-    default: return fail(NN_E_UNSUPPORTED, "component count not compiled into this library (3, 4, 6)");
+    case 8: return f(std::integral_constant<int, 8>{});
+    default:
+        return fail(NN_E_UNSUPPORTED,
+                    "component count not compiled into this library (3, 4, 5, 6, 8)");
     }
 }
 
 
 // Work-queue counters reset by a kernel rather than hipMemsetAsync, so that
 // the launch sequence stays valid inside a captured HIP graph.
-__global__ void resetQueueKernel(WorkQueue *q, unsigned int deferAll)
+__global__ void resetQueueKernel(WorkQueue *q, unsigned int deferAll, unsigned int claim)
 {
     q->head = 0u;
     q->deferred = 0u;
     q->coopHead = 0u;
     q->deferAll = deferAll;
+    q->claim = claim;
+    q->pending = 0u;
+    q->trialHead = 0u;
+    q->replayHead = 0u;
 }
 
 void releaseEvents(nn_context *ctx)
@@ -173,6 +190,24 @@ struct LaunchTimer
 };
 
 // Makes room for n deferred indices (synchronous allocation when it grows).
+// Pending list and stability records for n cells of `bytes` each (synchronous when it grows).
+int ensureRecords(nn_context *ctx, int64_t n, size_t bytes)
+{
+    if (n <= ctx->recordCap && bytes == ctx->recordBytes)
+        return NN_OK;
+    if (ctx->dPending) (void)hipFree(ctx->dPending);
+    if (ctx->dRecords) (void)hipFree(ctx->dRecords);
+    ctx->dPending = nullptr;
+    ctx->dRecords = nullptr;
+    ctx->recordCap = 0;
+    if (hipMalloc(&ctx->dPending, sizeof(unsigned int) * size_t(n)) != hipSuccess ||
+        hipMalloc(&ctx->dRecords, bytes * size_t(n)) != hipSuccess)
+        return fail(NN_E_OUT_OF_MEMORY, "device allocation of the stability records failed");
+    ctx->recordCap = n;
+    ctx->recordBytes = bytes;
+    return NN_OK;
+}
+
 int ensureList(nn_context *ctx, int64_t n)
 {
     if (n <= ctx->listCap)
@@ -244,16 +279,24 @@ hipError_t sortOrder(nn_context *ctx, int64_t n, hipStream_t st, const unsigned 
 
 unsigned int coopGrid(const nn_context *ctx)
 {
-    return static_cast<unsigned int>(ctx->coopBlocks > 0 ? ctx->coopBlocks : int64_t(ctx->numCUs) * 4);
+    return static_cast<unsigned int>(ctx->coopBlocks > 0
+                                         ? ctx->coopBlocks
+                                         : int64_t(ctx->numCUs) * ctx->sched.coop_blocks_per_cu);
 }
 
 unsigned int persistentGrid(const nn_context *ctx, int64_t n)
 {
-    // 16 wavefront-blocks per CU keeps every SIMD busy while the queue drains.
-    const int64_t cap = ctx->maxBlocks > 0 ? ctx->maxBlocks : int64_t(ctx->numCUs) * 16;
+    // Default 16 wavefront-blocks per CU keeps every SIMD busy while the queue drains.
+    const int64_t cap = ctx->maxBlocks > 0
+                            ? ctx->maxBlocks
+                            : int64_t(ctx->numCUs) * ctx->sched.persistent_blocks_per_cu;
     const int64_t need = (n + kBlock - 1) / kBlock;
     return static_cast<unsigned int>(std::max<int64_t>(1, std::min(cap, need)));
 }
+
+// 32-bit work-queue positions: the batch plus the overshoot of the last claims
+// (at most one 64-item tile per block) must stay below 2^32.
+constexpr int64_t kMaxQueueItems = int64_t(0xF0000000u);
 
 FluidParams toParams(const nn_fluid_desc &d)
 {
@@ -406,7 +449,11 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
     if (const char *env = std::getenv("NN_COOP_BLOCKS"))
         ctx->coopBlocks = std::max(0L, std::atol(env));
     if (const char *env = std::getenv("NN_ORDER"))
-        ctx->ordering = std::atoi(env) != 0;
+        ctx->sched.ordered = std::atoi(env) != 0 ? 1 : 0;
+    if (const char *env = std::getenv("NN_WAVE_CLAIM"))
+        ctx->sched.wave_claim = std::atoi(env) != 0 ? 1 : 0;
+    if (const char *env = std::getenv("NN_STAB_LOCKSTEP"))
+        ctx->lockstep = std::atoi(env) != 0;
     if (const char *env = std::getenv("NN_COOP_ALL"))
         ctx->coopAll = std::atoi(env) != 0;
     nn_flash_options defaults;
@@ -462,6 +509,8 @@ int nn_context_destroy(nn_context *ctx)
     if (ctx->dOpts) (void)hipFree(ctx->dOpts);
     if (ctx->dQueue) (void)hipFree(ctx->dQueue);
     if (ctx->dList) (void)hipFree(ctx->dList);
+    if (ctx->dPending) (void)hipFree(ctx->dPending);
+    if (ctx->dRecords) (void)hipFree(ctx->dRecords);
     releaseSort(ctx);
     delete ctx;
     return NN_OK;
@@ -491,7 +540,7 @@ int nn_eos_phase_batch(nn_context *ctx, int64_t n, const double *d_P, const doub
 
 static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
                        const double *d_z, const uint8_t *d_allowed, const double *d_seed,
-                       const nn_flash_out *out, void *stream)
+                       FlashHint hint, const nn_flash_out *out, void *stream)
 {
     if (!ctx || n < 0 || !out)
         return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
@@ -501,31 +550,47 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
         !out->iterations || !out->beta || !out->saturation || !out->x || !out->Z ||
         !out->rho_m || !out->k_go || !out->k_wo)
         return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
-    if (n > int64_t(0xFFFFFFF0u))
+    if (n > kMaxQueueItems)
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const nn_flash_out o = *out;
+    if (ctx->sched.plain)
+    {
+        // Baseline form: one kernel, one work-item per state, index order.
+        return dispatchNc(ctx->host.nc, [&](auto ncTag) {
+            constexpr int NC = decltype(ncTag)::value;
+            const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+            LaunchTimer t(ctx, NN_KERNEL_FLASH, st);
+            NN_HIP((Launch<NC>::flash(grid, 0, st, ctx->dFluid, ctx->dOpts, n, d_P, d_T, d_z,
+                                      d_allowed, d_seed, hint, o, nullptr, nullptr, nullptr, true,
+                                      nullptr, nullptr)));
+            t.done();
+            return NN_OK;
+        });
+    }
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
-    const bool ordered = ctx->ordering && n >= kOrderMinItems;
+    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems;
     if (ordered)
         if (const int rc = ensureSort(ctx, n); rc != NN_OK)
             return rc;
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    const nn_flash_out o = *out;
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue, ctx->coopAll ? 1u : 0u);
+        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue, ctx->coopAll ? 1u : 0u,
+                                          ctx->sched.wave_claim ? 1u : 0u);
         NN_HIP(hipGetLastError());
         const unsigned int *order = nullptr;
         if (ordered)
         {
-            NN_HIP((Launch<NC>::flashKeys(st, n, d_P, d_T, d_z, ctx->dKeys[0], ctx->dIdx[0])));
+            NN_HIP((Launch<NC>::flashKeys(st, n, d_P, d_T, d_z, hint.hint, ctx->dKeys[0],
+                                          ctx->dIdx[0])));
             NN_HIP(sortOrder(ctx, n, st, &order));
         }
         LaunchTimer t(ctx, NN_KERNEL_FLASH, st);
         t.makeSplit();
         NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), coopGrid(ctx), st, ctx->dFluid,
-                                  ctx->dOpts, n, d_P, d_T, d_z, d_allowed, d_seed, o, ctx->dQueue,
-                                  ctx->dList, order, t.splitA, t.splitB)));
+                                  ctx->dOpts, n, d_P, d_T, d_z, d_allowed, d_seed, hint, o,
+                                  ctx->dQueue, ctx->dList, order, false, t.splitA, t.splitB)));
         t.doneSplit(NN_KERNEL_FLASH_COOP);
         return NN_OK;
     });
@@ -534,7 +599,115 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
 int nn_flash_batch(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
                    const double *d_z, const nn_flash_out *out, void *stream)
 {
-    return launchFlash(ctx, n, d_P, d_T, d_z, nullptr, nullptr, out, stream);
+    return launchFlash(ctx, n, d_P, d_T, d_z, nullptr, nullptr, FlashHint{nullptr, nullptr}, out,
+                       stream);
+}
+
+int nn_flash_seeded_batch(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
+                          const double *d_z, const uint8_t *d_hint, const double *d_log_k,
+                          const nn_flash_out *out, void *stream)
+{
+    if (!d_hint && n > 0)
+        return fail(NN_E_INVALID_ARGUMENT, "hint array required");
+    return launchFlash(ctx, n, d_P, d_T, d_z, nullptr, nullptr, FlashHint{d_hint, d_log_k}, out,
+                       stream);
+}
+
+int nn_context_get_schedule(nn_context *ctx, nn_schedule *s)
+{
+    if (!ctx || !s)
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    *s = ctx->sched;
+    return NN_OK;
+}
+
+int nn_context_set_schedule(nn_context *ctx, const nn_schedule *s)
+{
+    if (!ctx || !s)
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    if (s->persistent_blocks_per_cu < 1 || s->persistent_blocks_per_cu > 64 ||
+        s->coop_blocks_per_cu < 1 || s->coop_blocks_per_cu > 64)
+        return fail(NN_E_INVALID_ARGUMENT, "blocks per CU must be in 1..64");
+    ctx->sched = *s;
+    ctx->sched.ordered = s->ordered ? 1 : 0;
+    ctx->sched.wave_claim = s->wave_claim ? 1 : 0;
+    ctx->sched.plain = s->plain ? 1 : 0;
+    ctx->maxBlocks = 0;
+    ctx->coopBlocks = 0;
+    return NN_OK;
+}
+
+int nn_flash_autotune(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
+                      const double *d_z, const nn_flash_out *out, int32_t repeats,
+                      nn_schedule *best, double *best_ms, void *stream)
+{
+    if (!ctx || n <= 0 || !out || repeats < 1)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const nn_schedule saved = ctx->sched;
+    const int64_t savedMax = ctx->maxBlocks, savedCoop = ctx->coopBlocks;
+    const bool savedTiming = ctx->timing;
+    ctx->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    NN_HIP(hipEventCreate(&e0));
+    NN_HIP(hipEventCreate(&e1));
+    nn_schedule winner = saved;
+    double winnerMs = 1.0e300;
+    int rc = NN_OK;
+    // Candidates: processing order x claim granularity x persistent grid; the
+    // cooperative grid only serves the few deferred states and stays at 4/CU.
+    const int grids[4] = {4, 8, 16, 32};
+    for (int ordered = 1; ordered >= 0 && rc == NN_OK; --ordered)
+        for (int claim = 0; claim <= 1 && rc == NN_OK; ++claim)
+            for (int g = 0; g < 4 && rc == NN_OK; ++g)
+            {
+                const nn_schedule cand{grids[g], 4, ordered, claim, 0};
+                ctx->sched = cand;
+                ctx->maxBlocks = ctx->coopBlocks = 0;
+                double bestRun = 1.0e300;
+                for (int r = 0; r < repeats && rc == NN_OK; ++r)
+                {
+                    if (hipEventRecord(e0, st) != hipSuccess)
+                    {
+                        rc = hipFail(hipGetLastError(), "hipEventRecord");
+                        break;
+                    }
+                    rc = launchFlash(ctx, n, d_P, d_T, d_z, nullptr, nullptr,
+                                     FlashHint{nullptr, nullptr}, out, stream);
+                    if (rc != NN_OK)
+                        break;
+                    float ms = 0.0f;
+                    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    {
+                        rc = hipFail(hipGetLastError(), "autotune timing");
+                        break;
+                    }
+                    bestRun = std::min(bestRun, double(ms));
+                }
+                if (rc == NN_OK && bestRun < winnerMs)
+                {
+                    winnerMs = bestRun;
+                    winner = cand;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->timing = savedTiming;
+    if (rc != NN_OK)
+    {
+        ctx->sched = saved;
+        ctx->maxBlocks = savedMax;
+        ctx->coopBlocks = savedCoop;
+        return rc;
+    }
+    ctx->sched = winner;
+    ctx->maxBlocks = ctx->coopBlocks = 0;
+    if (best)
+        *best = winner;
+    if (best_ms)
+        *best_ms = winnerMs;
+    return NN_OK;
 }
 
 int nn_flash_restricted_batch(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
@@ -543,7 +716,8 @@ int nn_flash_restricted_batch(nn_context *ctx, int64_t n, const double *d_P, con
 {
     if (!d_allowed)
         return fail(NN_E_INVALID_ARGUMENT, "allowed masks required");
-    return launchFlash(ctx, n, d_P, d_T, d_z, d_allowed, d_seed, out, stream);
+    return launchFlash(ctx, n, d_P, d_T, d_z, d_allowed, d_seed, FlashHint{nullptr, nullptr}, out,
+                       stream);
 }
 
 int nn_stability_batch(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
@@ -583,18 +757,19 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
         return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
     if (ctx->host.water < 0)
         return fail(NN_E_INVALID_ARGUMENT, "three-phase update needs a water component");
-    if (n > int64_t(0xFFFFFFF0u))
+    if (n > kMaxQueueItems)
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
-    const bool ordered = ctx->ordering && n >= kOrderMinItems;
+    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems;
     if (ordered)
         if (const int rc = ensureSort(ctx, n); rc != NN_OK)
             return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue, ctx->coopAll ? 1u : 0u);
+        resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue, ctx->coopAll ? 1u : 0u,
+                                          ctx->sched.wave_claim ? 1u : 0u);
         NN_HIP(hipGetLastError());
         const unsigned int *order = nullptr;
         if (ordered)
@@ -603,12 +778,17 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
                                                 d_phase_state, ctx->dKeys[0], ctx->dIdx[0])));
             NN_HIP(sortOrder(ctx, n, st, &order));
         }
+        if (ctx->lockstep)
+            if (const int rc = ensureRecords(ctx, n, sizeof(StabRecord<NC>)); rc != NN_OK)
+                return rc;
         LaunchTimer t(ctx, NN_KERNEL_PHASE_UPDATE, st);
         t.makeSplit();
         NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, persistentGrid(ctx, n),
                                         coopGrid(ctx), st, ctx->dFluid, ctx->dOpts, n, d_T,
                                         d_primary, d_phase_state, d_status, d_flags, ctx->dQueue,
-                                        ctx->dList, order, t.splitA, t.splitB)));
+                                        ctx->dList, order, ctx->lockstep ? ctx->dPending : nullptr,
+                                        ctx->lockstep ? ctx->dRecords : nullptr, t.splitA,
+                                        t.splitB)));
         t.doneSplit(NN_KERNEL_PHASE_UPDATE_COOP);
         return NN_OK;
     });
@@ -708,6 +888,18 @@ struct nn_assembly
 };
 
 extern "C" {
+
+int nn_assembly_info(const nn_assembly *a, nn_context **ctx, int64_t *ncell, int64_t *nnzb,
+                     int32_t *np)
+{
+    if (!a)
+        return fail(NN_E_INVALID_ARGUMENT, "null assembly");
+    if (ctx) *ctx = a->ctx;
+    if (ncell) *ncell = a->view.ncell;
+    if (nnzb) *nnzb = a->nnzb;
+    if (np) *np = 3 * a->ctx->host.nc + 1 + (a->hasWell ? 1 : 0);
+    return NN_OK;
+}
 
 int nn_assembly_destroy(nn_assembly *a)
 {

@@ -77,6 +77,10 @@ struct PhaseUpdater
     Err &err;
     Flash<NC, Mode> flash;
     double T;
+    // Record/replay of the stability test this update runs directly on its
+    // active set (the common path: no phase removed, fewer than three phases).
+    int stabMode = kStabNone;
+    StabRecord<NC> *stabRec = nullptr;
 
     __host__ __device__ PhaseUpdater(const FluidParams &fp, const FlashOptions &op, Err &e, double temp)
         : f(fp), o(op), err(e), flash(fp, op, e), T(temp)
@@ -475,7 +479,10 @@ struct PhaseUpdater
         }
         else if (popcount3(active) < 3)
         {
+            flash.stabMode = stabMode;
+            flash.stabRec = stabRec;
             const StabilityResult<NC> st = flash.stabilityTest(P, T, overall, active, comps);
+            flash.stabMode = kStabNone;
             if (err.bad())
                 return flags;
             const bool invalid = !st.valid;
@@ -563,13 +570,16 @@ __host__ __device__ void encodePhaseState(const PhaseStateData<NC> &ps, double *
     v[L::suppression] = double(ps.suppression);
 }
 
-// One cell of nn_phase_update_batch (natural_petsc_runtime_state.inc:215-221).
-// Returns false when a kModeDefer run handed the cell to the cooperative pass
-// (nothing written); in kModeCoop only lane 0 writes.
+// One cell of nn_phase_update_batch (natural_petsc_runtime_state.inc:215-221):
+// kItemDone, kItemDeferred (a kModeDefer run handed the cell to the cooperative
+// pass) or kItemPending (kStabRecord: its stability-test inputs are in rec; the
+// lockstep trial pass and a kStabReplay run finish it).  Deferred and pending
+// cells leave primary/phaseState untouched; in kModeCoop only lane 0 writes.
 template <int NC, bool Well, int Mode = kModePlain>
-__host__ __device__ bool phaseUpdateCell(const FluidParams &f, const FlashOptions &o, double T,
-                                         double *primary, double *phaseState, int64_t s,
-                                         int32_t *status, uint8_t *flags)
+__host__ __device__ int phaseUpdateCell(const FluidParams &f, const FlashOptions &o, double T,
+                                        double *primary, double *phaseState, int64_t s,
+                                        int32_t *status, uint8_t *flags,
+                                        int stabMode = kStabNone, StabRecord<NC> *rec = nullptr)
 {
     using L = Layout<NC, Well>;
     const bool writer = Mode != kModeCoop || waveLane() == 0;
@@ -586,31 +596,35 @@ __host__ __device__ bool phaseUpdateCell(const FluidParams &f, const FlashOption
             status[s] = NN_STATUS_THERMO_ERROR;
             if (flags) flags[s] = 0;
         }
-        return true;
+        return kItemDone;
     }
     Err err;
     PhaseUpdater<NC, Well, Mode> up(f, o, err, T);
+    up.stabMode = rec ? stabMode : kStabNone;
+    up.stabRec = rec ? rec + s : nullptr;
     up.sanitize(prim);
     const int packed = up.update(prim, ps);
     const int st = packed >> 8;
     const uint8_t fl = uint8_t(packed & 0xff);
     if (err.code == kErrDefer)
-        return false;
+        return kItemDeferred;
+    if (err.code == kErrPending)
+        return kItemPending;
     if (!writer)
-        return true;
+        return kItemDone;
     if (err.bad())
     {
         // The reference throws out of updatePhaseState; inputs stay untouched.
         status[s] = err.code;
         if (flags) flags[s] = fl;
-        return true;
+        return kItemDone;
     }
     encodePhaseState<NC, Well>(ps, psv);
     for (int k = 0; k < L::NP; ++k) primary[s * L::NP + k] = prim[k];
     for (int k = 0; k < L::NPS; ++k) phaseState[s * L::NPS + k] = psv[k];
     status[s] = st;
     if (flags) flags[s] = fl;
-    return true;
+    return kItemDone;
 }
 
 } // namespace nn

@@ -101,6 +101,26 @@ class Context:
         _abi.check(self.lib.nn_context_set_kernel_timing(self.handle, int(bool(enable))),
                    "nn_context_set_kernel_timing")
 
+    def schedule(self) -> dict:
+        """Launch shapes of this context's flash-like batches (nn_schedule)."""
+        s = _abi.Schedule()
+        _abi.check(self.lib.nn_context_get_schedule(self.handle, ctypes.byref(s)),
+                   "nn_context_get_schedule")
+        return s.as_dict()
+
+    def set_schedule(self, **fields) -> dict:
+        """Change some nn_schedule fields (persistent_blocks_per_cu, coop_blocks_per_cu,
+        ordered, wave_claim, plain); returns the schedule now in force."""
+        cur = self.schedule()
+        unknown = set(fields) - set(cur)
+        if unknown:
+            raise KeyError(f"unknown schedule fields {sorted(unknown)}")
+        cur.update({k: int(v) for k, v in fields.items()})
+        s = _abi.Schedule(**cur)
+        _abi.check(self.lib.nn_context_set_schedule(self.handle, ctypes.byref(s)),
+                   "nn_context_set_schedule")
+        return self.schedule()
+
     def kernel_time(self, kind: str) -> tuple[float, int]:
         """(summed milliseconds, launches) of one kernel kind since timing was enabled."""
         ms = ctypes.c_double()
@@ -138,6 +158,34 @@ class ThreePhaseFlash(Context):
         _abi.check(self.lib.nn_flash_batch(self.handle, n, _ptr(P), _ptr(T), _ptr(z),
                                            ctypes.byref(abi), _stream()), "nn_flash_batch")
         return out
+
+    def flash_seeded(self, P, T, z, hint, log_k=None,
+                     out: FlashBatchResult | None = None) -> FlashBatchResult:
+        """flash with learned-initialiser hints (NN_HINT_* per state, log-K seeds n x nc)."""
+        P, T, z = _dev(P, torch.float64), _dev(T, torch.float64), _dev(z, torch.float64)
+        hint = _dev(hint, torch.uint8)
+        log_k = _dev(log_k, torch.float64) if log_k is not None else None
+        n = P.shape[0]
+        out = out or FlashBatchResult.empty(n, self.nc, P.device)
+        abi = out.abi()
+        _abi.check(self.lib.nn_flash_seeded_batch(self.handle, n, _ptr(P), _ptr(T), _ptr(z),
+                                                  _ptr(hint), _ptr(log_k), ctypes.byref(abi),
+                                                  _stream()), "nn_flash_seeded_batch")
+        return out
+
+    def autotune(self, P, T, z, repeats: int = 2) -> tuple[dict, float]:
+        """nn_flash_autotune on a sample batch: keeps and returns the fastest schedule
+        and its time in ms."""
+        P, T, z = _dev(P, torch.float64), _dev(T, torch.float64), _dev(z, torch.float64)
+        n = P.shape[0]
+        out = FlashBatchResult.empty(n, self.nc, P.device)
+        abi = out.abi()
+        best = _abi.Schedule()
+        ms = ctypes.c_double()
+        _abi.check(self.lib.nn_flash_autotune(self.handle, n, _ptr(P), _ptr(T), _ptr(z),
+                                              ctypes.byref(abi), int(repeats), ctypes.byref(best),
+                                              ctypes.byref(ms), _stream()), "nn_flash_autotune")
+        return best.as_dict(), ms.value
 
     def flash_restricted(self, P, T, z, allowed, seed=None) -> FlashBatchResult:
         """CubicThreePhaseFlash::flashRestricted (optionally with composition seeds)."""

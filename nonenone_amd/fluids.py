@@ -4,6 +4,7 @@ Component data follow the reference's case tables:
   H2O, CO2, nC10   case/h2o_co2_nc10_2d_benchmark/case_config.hpp:49-58
   C1 (CH4)         case/3p4c_pr_reservoir/case_config.hpp:90-99
   nC4, nC16        case/scw_co2_nc4_nc16_3d_migration/case_config.hpp:46-55
+  N2, C2, C3       case/panfili2025_case3_fullphysics/case_config.hpp:111-147
 EOS constants (Omega_a = 0.45724, Omega_b = 0.07780, eos type 5 = PR78 m(omega)
 for omega > 0.49, m1 = 1+sqrt2, m2 = 1-sqrt2) follow case_config.hpp:60-63,77.
 Soreide-Whitson aqueous H2O-solute k_ij: CO2 -0.06609, nC10 -0.14229
@@ -20,6 +21,7 @@ from __future__ import annotations
 from ._abi import FluidDesc, NN_MAX_NC, NN_MODEL_CPA, NN_MODEL_PR, NN_MODEL_SW
 
 R = 8.31446261815324
+_PSI = (0.45359237 * 9.80665) / (0.0254 * 0.0254)   # common/include/common/units.hpp
 
 # name: (Tc [K], Pc [Pa], Vc [m3/mol], omega, M [kg/mol])
 COMPONENTS = {
@@ -29,6 +31,14 @@ COMPONENTS = {
     "nC4": (425.125, 3.7960e6, 2.5492e-4, 0.20081, 0.0581222),
     "nC10": (617.70, 2.103e6, 6.10e-4, 0.4920, 0.14228168),
     "nC16": (723.0, 1.410e6, 9.0e-4, 0.742, 0.2264412),
+    # N2, C2, C3: Tc, Pc, omega, M of case/panfili2025_case3_fullphysics/case_config.hpp:111-147
+    # (degR / psia converted); Vc from the Zc = 0.27 estimate that case uses.
+    "N2": (227.16 * 5.0 / 9.0, 492.31 * _PSI, 0.27 * R * 227.16 * 5.0 / 9.0 / (492.31 * _PSI),
+           0.04000, 0.028013),
+    "C2": (549.77 * 5.0 / 9.0, 708.34 * _PSI, 0.27 * R * 549.77 * 5.0 / 9.0 / (708.34 * _PSI),
+           0.09860, 0.03007),
+    "C3": (665.64 * 5.0 / 9.0, 618.70 * _PSI, 0.27 * R * 665.64 * 5.0 / 9.0 / (618.70 * _PSI),
+           0.15240, 0.044097),
 }
 
 # Pairwise k_ij used for the six-component mixture (symmetric).
@@ -99,6 +109,41 @@ def six_component(model: str = "PR") -> FluidDesc:
     for (a, b), v in _PAIR_KIJ_NC6.items():
         i, j = names.index(a), names.index(b)
         kij[i][j] = kij[j][i] = v
+    return make_desc(names, kij, model=model, eos_type=5 if model == "PR" else 1)
+
+
+def five_component(model: str = "PR") -> FluidDesc:
+    """H2O, CO2, C1, nC4, nC10 (PR78 / SW), k_ij as the six-component preset."""
+    names = ["H2O", "CO2", "C1", "nC4", "nC10"]
+    nc = len(names)
+    kij = [[0.0] * nc for _ in range(nc)]
+    for (a, b), v in _PAIR_KIJ_NC6.items():
+        if a in names and b in names:
+            i, j = names.index(a), names.index(b)
+            kij[i][j] = kij[j][i] = v
+    return make_desc(names, kij, model=model, eos_type=5 if model == "PR" else 1)
+
+
+# Pairwise k_ij of the eight-component preset (symmetric; unlisted pairs are 0).
+_PAIR_KIJ_NC8 = dict(_PAIR_KIJ_NC6)
+_PAIR_KIJ_NC8.update({
+    ("H2O", "N2"): 0.4778, ("H2O", "C2"): 0.4920, ("H2O", "C3"): 0.5525,
+    ("CO2", "N2"): -0.0170, ("CO2", "C2"): 0.1300, ("CO2", "C3"): 0.1350,
+    ("N2", "C1"): 0.0311, ("N2", "C2"): 0.0515, ("N2", "C3"): 0.0852,
+    ("N2", "nC4"): 0.0800, ("N2", "nC10"): 0.1100,
+})
+
+
+def eight_component(model: str = "PR") -> FluidDesc:
+    """H2O, CO2, N2, C1, C2, C3, nC4, nC10 (PR78 / SW): a synthetic eight-component
+    preset for the serving workload (N2/C2/C3 data of the Panfili case)."""
+    names = ["H2O", "CO2", "N2", "C1", "C2", "C3", "nC4", "nC10"]
+    nc = len(names)
+    kij = [[0.0] * nc for _ in range(nc)]
+    for (a, b), v in _PAIR_KIJ_NC8.items():
+        if a in names and b in names:
+            i, j = names.index(a), names.index(b)
+            kij[i][j] = kij[j][i] = v
     return make_desc(names, kij, model=model, eos_type=5 if model == "PR" else 1)
 
 
