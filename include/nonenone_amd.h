@@ -191,6 +191,11 @@ int nn_flash_batch(nn_context *ctx, int64_t n,
 #define NN_HINT_NONE 0
 #define NN_HINT_SINGLE_PHASE 1
 #define NN_HINT_LOG_K 2
+/* Bits 2-3 of a hint: predicted phase count - 1, used only to order the batch
+ * (the solver path is the reference's); nn_mlp_hints sets them with modes bit
+ * NN_HINT_ORDER.  A batch whose hints carry only these bits gives results
+ * bit-identical to nn_flash_batch. */
+#define NN_HINT_ORDER 4
 int nn_flash_seeded_batch(nn_context *ctx, int64_t n,
                           const double *d_P, const double *d_T, const double *d_z,
                           const uint8_t *d_hint, const double *d_log_k,
@@ -413,7 +418,9 @@ int nn_mlp_predict(nn_mlp *m, int64_t n, const float *d_X, float *d_prob, float 
                    void *stream);
 /* Flash hints from predictions: NN_HINT_SINGLE_PHASE where p(1 phase) >= threshold
  * and modes has that bit, else NN_HINT_LOG_K where p(2) or p(3) >= threshold and
- * modes has that bit, else NN_HINT_NONE; d_seed_log_k (n x nc FP64) = predicted log-K. */
+ * modes has that bit, else NN_HINT_NONE; with NN_HINT_ORDER in modes, bits 2-3
+ * carry the most probable phase count - 1; d_seed_log_k (n x nc FP64) =
+ * predicted log-K. */
 int nn_mlp_hints(nn_mlp *m, int64_t n, const float *d_prob, const float *d_logk,
                  float threshold, int32_t modes, uint8_t *d_hint, double *d_seed_log_k,
                  void *stream);

@@ -133,6 +133,7 @@ class Schedule(ctypes.Structure):
 NN_HINT_NONE = 0
 NN_HINT_SINGLE_PHASE = 1
 NN_HINT_LOG_K = 2
+NN_HINT_ORDER = 4
 NN_MLP_MAX_BATCH = 1 << 20
 
 _PRODUCT_LIB = None

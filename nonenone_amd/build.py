@@ -28,7 +28,7 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libnonenone_amd.so"
 BUILD_DIR = HERE.parent / "_build"
 ARCH = os.environ.get("NN_OFFLOAD_ARCH", "gfx950")
-UNITS = ("nn_api.hip", "nn_nc3.hip", "nn_nc4.hip", "nn_nc5.hip", "nn_nc6.hip", "nn_nc8.hip",
+UNITS = ("nn_api.hip", "nn_nc2.hip", "nn_nc3.hip", "nn_nc4.hip", "nn_nc5.hip", "nn_nc6.hip", "nn_nc8.hip",
          "mlp.hip", "serve.hip")
 FLAGS = ["-std=c++17", "-O2", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -7,6 +7,8 @@
 //   association strength Delta_ij           :1561-1582
 //   site fractions: analytic symmetric      :1606-1623
 //                   single-donor Newton     :1624-1730
+//                   general dense Newton +
+//                   damped fixed point      :1731-2031 (dense solve :1233-1302)
 //   pressure at density (SRK / PR term)     :2050-2078
 //   density root (bracket + bisection)      :2090-2178
 //   density root with two implicit-function
@@ -19,9 +21,7 @@
 // as AD scalars (temperature, composition, mixing sums) are S; per-temperature
 // constants, which carry no derivatives there, are double here — products and
 // sums with them round identically, and every division by a temperature
-// quantity keeps an S divisor.  The general multi-donor association solver
-// (:1731-2031) is not built; fluids that need it are rejected at context
-// creation.
+// quantity keeps an S divisor.
 #pragma once
 
 #include "common.hpp"
@@ -165,7 +165,277 @@ __host__ __device__ S cpaDelta(const CpaTemp<NC> &c, int i, int j, const S &g)
     return g * tf * bij * beta;
 }
 
-// :1584-2048 (analytic symmetric and single-donor solvers)
+// Gaussian elimination with partial pivoting (by value magnitude) and back
+// substitution on the leading n x n block, the site Newton step's dense solve
+// (:1233-1302); false on a (near-)singular pivot or a non-finite solution.
+template <int M, class S>
+__host__ __device__ bool cpaSolveDense(S (&a)[M][M], S (&b)[M], int n, S (&x)[M])
+{
+    for (int col = 0; col < n; ++col)
+    {
+        int piv = col;
+        double mag = ::fabs(sval(a[col][col]));
+        for (int r = col + 1; r < n; ++r)
+        {
+            const double v = ::fabs(sval(a[r][col]));
+            if (v > mag)
+            {
+                piv = r;
+                mag = v;
+            }
+        }
+        if (!finite(mag) || mag <= 1.0e-18)
+            return false;
+        if (piv != col)
+        {
+            for (int j = 0; j < M; ++j)
+            {
+                const S t = a[piv][j];
+                a[piv][j] = a[col][j];
+                a[col][j] = t;
+            }
+            const S t = b[piv];
+            b[piv] = b[col];
+            b[col] = t;
+        }
+        const S diag = a[col][col];
+        for (int r = col + 1; r < n; ++r)
+        {
+            const S factor = a[r][col] / diag;
+            a[r][col] = S(0.0);
+            for (int j = col + 1; j < n; ++j) a[r][j] -= factor * a[col][j];
+            b[r] -= factor * b[col];
+        }
+    }
+    for (int r = n - 1; r >= 0; --r)
+    {
+        S v = b[r];
+        for (int c = r + 1; c < n; ++c) v -= a[r][c] * x[c];
+        const S d = a[r][r];
+        const double dv = ::fabs(sval(d));
+        if (!finite(dv) || dv <= 1.0e-18)
+            return false;
+        x[r] = v / d;
+        if (!finite(sval(x[r])))
+            return false;
+    }
+    return true;
+}
+
+// General multi-donor / multi-acceptor site fractions (:1731-2031): couplings
+// D_ij = rho x_j A_j Delta_ij (donor i, acceptor j) and C_ji = rho x_i D_i
+// Delta_ij; one Jacobi sweep from X = 1; dense Newton on the mass-action
+// residuals X (1 + sum C X') - 1 with a halving line search that keeps
+// 0 < X <= 1 and lowers the residual; damped fixed point from X = 1 when
+// Newton fails; thermo error when that does not converge either.
+template <int NC, class S>
+__host__ __device__ void cpaGeneralSites(const FluidParams &f, const CpaTemp<NC> &c, const S &rho,
+                                         const S *x, CpaAssoc<NC, S> &st, Err &err)
+{
+    constexpr int M = 2 * NC;
+    int dU[NC], aU[NC];
+    int count = 0;
+    for (int i = 0; i < NC; ++i) dU[i] = f.cpaDonors[i] > 0 ? count++ : -1;
+    for (int i = 0; i < NC; ++i) aU[i] = f.cpaAcceptors[i] > 0 ? count++ : -1;
+    S dC[NC][NC], aC[NC][NC];
+    for (int i = 0; i < NC; ++i)
+        for (int j = 0; j < NC; ++j)
+        {
+            dC[i][j] = S(0.0);
+            aC[i][j] = S(0.0);
+        }
+    for (int i = 0; i < NC; ++i)
+    {
+        if (f.cpaDonors[i] <= 0)
+            continue;
+        for (int j = 0; j < NC; ++j)
+        {
+            if (f.cpaAcceptors[j] <= 0)
+                continue;
+            const S delta = cpaDelta<NC, S>(c, i, j, st.g);
+            dC[i][j] = rho * x[j] * double(f.cpaAcceptors[j]) * delta;
+            aC[j][i] = rho * x[i] * double(f.cpaDonors[i]) * delta;
+        }
+    }
+    for (int i = 0; i < NC; ++i)
+    {
+        if (f.cpaDonors[i] > 0)
+        {
+            S sum = S(0.0);
+            for (int j = 0; j < NC; ++j) sum += dC[i][j];
+            st.donorX[i] = S(1.0) / (S(1.0) + sum);
+        }
+        if (f.cpaAcceptors[i] > 0)
+        {
+            S sum = S(0.0);
+            for (int j = 0; j < NC; ++j) sum += aC[i][j];
+            st.acceptorX[i] = S(1.0) / (S(1.0) + sum);
+        }
+    }
+    auto residualNorm = [&](const S *dX, const S *aX) {
+        double norm = 0.0;
+        for (int i = 0; i < NC; ++i)
+        {
+            if (f.cpaDonors[i] > 0)
+            {
+                S den = S(1.0);
+                for (int j = 0; j < NC; ++j) den += dC[i][j] * aX[j];
+                norm = maxd(norm, ::fabs(sval(dX[i] * den - 1.0)));
+            }
+            if (f.cpaAcceptors[i] > 0)
+            {
+                S den = S(1.0);
+                for (int j = 0; j < NC; ++j) den += aC[i][j] * dX[j];
+                norm = maxd(norm, ::fabs(sval(aX[i] * den - 1.0)));
+            }
+        }
+        return norm;
+    };
+    bool converged = false;
+    const int maxNewton = f.cpaMaxIter < 64 ? f.cpaMaxIter : 64;
+    for (int it = 0; it < maxNewton; ++it)
+    {
+        S jac[M][M], rhs[M], inc[M];
+        for (int r = 0; r < M; ++r)
+        {
+            rhs[r] = S(0.0);
+            inc[r] = S(0.0);
+            for (int k = 0; k < M; ++k) jac[r][k] = S(0.0);
+        }
+        double norm = 0.0;
+        for (int i = 0; i < NC; ++i)
+        {
+            const int row = dU[i];
+            if (row < 0)
+                continue;
+            S den = S(1.0);
+            for (int j = 0; j < NC; ++j)
+            {
+                den += dC[i][j] * st.acceptorX[j];
+                if (aU[j] >= 0)
+                    jac[row][aU[j]] = st.donorX[i] * dC[i][j];
+            }
+            const S residual = st.donorX[i] * den - 1.0;
+            rhs[row] = -residual;
+            jac[row][row] = den;
+            norm = maxd(norm, ::fabs(sval(residual)));
+        }
+        for (int i = 0; i < NC; ++i)
+        {
+            const int row = aU[i];
+            if (row < 0)
+                continue;
+            S den = S(1.0);
+            for (int j = 0; j < NC; ++j)
+            {
+                den += aC[i][j] * st.donorX[j];
+                if (dU[j] >= 0)
+                    jac[row][dU[j]] = st.acceptorX[i] * aC[i][j];
+            }
+            const S residual = st.acceptorX[i] * den - 1.0;
+            rhs[row] = -residual;
+            jac[row][row] = den;
+            norm = maxd(norm, ::fabs(sval(residual)));
+        }
+        if (norm <= f.cpaTol)
+        {
+            converged = true;
+            break;
+        }
+        if (!cpaSolveDense<M, S>(jac, rhs, count, inc))
+            break;
+        bool accepted = false;
+        double damping = 1.0;
+        for (int ls = 0; ls < 20; ++ls)
+        {
+            S cd[NC], ca[NC];
+            bool physical = true;
+            for (int i = 0; i < NC; ++i)
+            {
+                cd[i] = st.donorX[i];
+                ca[i] = st.acceptorX[i];
+                if (dU[i] >= 0)
+                {
+                    cd[i] += damping * inc[dU[i]];
+                    const double v = sval(cd[i]);
+                    physical = physical && finite(v) && v > 0.0 && v <= 1.0;
+                }
+                if (aU[i] >= 0)
+                {
+                    ca[i] += damping * inc[aU[i]];
+                    const double v = sval(ca[i]);
+                    physical = physical && finite(v) && v > 0.0 && v <= 1.0;
+                }
+            }
+            if (physical)
+            {
+                const double cn = residualNorm(cd, ca);
+                if (finite(cn) && cn < norm)
+                {
+                    for (int i = 0; i < NC; ++i)
+                    {
+                        st.donorX[i] = cd[i];
+                        st.acceptorX[i] = ca[i];
+                    }
+                    accepted = true;
+                    if (cn <= f.cpaTol)
+                        converged = true;
+                    break;
+                }
+            }
+            damping *= 0.5;
+        }
+        if (converged || !accepted)
+            break;
+    }
+    if (converged)
+        return;
+    for (int i = 0; i < NC; ++i)
+    {
+        st.donorX[i] = S(1.0);
+        st.acceptorX[i] = S(1.0);
+    }
+    for (int it = 0; it < f.cpaMaxIter; ++it)
+    {
+        S nd[NC], na[NC];
+        double change = 0.0;
+        for (int i = 0; i < NC; ++i)
+        {
+            nd[i] = st.donorX[i];
+            na[i] = st.acceptorX[i];
+            if (f.cpaDonors[i] > 0)
+            {
+                S sum = S(0.0);
+                for (int j = 0; j < NC; ++j) sum += dC[i][j] * st.acceptorX[j];
+                const S target = S(1.0) / (S(1.0) + sum);
+                nd[i] = st.donorX[i] + f.cpaDamping * (target - st.donorX[i]);
+                change = maxd(change, ::fabs(sval(nd[i] - st.donorX[i])));
+            }
+            if (f.cpaAcceptors[i] > 0)
+            {
+                S sum = S(0.0);
+                for (int j = 0; j < NC; ++j) sum += aC[i][j] * st.donorX[j];
+                const S target = S(1.0) / (S(1.0) + sum);
+                na[i] = st.acceptorX[i] + f.cpaDamping * (target - st.acceptorX[i]);
+                change = maxd(change, ::fabs(sval(na[i] - st.acceptorX[i])));
+            }
+        }
+        for (int i = 0; i < NC; ++i)
+        {
+            st.donorX[i] = nd[i];
+            st.acceptorX[i] = na[i];
+        }
+        if (change <= f.cpaTol)
+            return;
+        if (it + 1 == f.cpaMaxIter)
+        {
+            err.raise(kErrThermo);  // "CPA association-site iteration did not converge."
+            return;
+        }
+    }
+}
+
+// :1584-2048 (analytic symmetric, single-donor and general solvers)
 template <int NC, class S>
 __host__ __device__ void cpaAssociation(const FluidParams &f, const CpaTemp<NC> &c, const S &rho,
                                         const S *x, const CpaMix<NC, S> &m,
@@ -192,6 +462,12 @@ __host__ __device__ void cpaAssociation(const FluidParams &f, const CpaTemp<NC> 
         const S X = S(2.0) / (S(1.0) + sqrt(S(1.0) + S(4.0) * A));
         st.donorX[k] = X;
         st.acceptorX[k] = X;
+    }
+    else if (scheme == kCpaGeneral)
+    {
+        cpaGeneralSites<NC, S>(f, c, rho, x, st, err);
+        if (err.bad())
+            return;
     }
     else if (scheme <= -2)
     {

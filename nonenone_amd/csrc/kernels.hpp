@@ -96,7 +96,7 @@ __device__ bool flashOne(const FluidParams &f, const FlashOptions &o, int64_t s,
     {
         if (h.hint)
         {
-            fl.hint = h.hint[s];
+            fl.hint = h.hint[s] & 3;  // bits 2-3 only order the batch
             fl.hintLogK = h.logK ? h.logK + s * NC : nullptr;
         }
         r = fl.flash(P[s], T[s], zs);
@@ -264,8 +264,9 @@ __global__ void flashKeyKernel(int64_t n, const double *__restrict__ P,
     k |= quantize(lp, 11.5, 18.5, 10) << 12;
     k |= quantize(z[s * NC], 0.0, 1.0, 6) << 6;
     k |= quantize(z[s * NC + 1], 0.0, 1.0, 6);
-    if (hint)  // learned hint first: states on the same solver path share wavefronts
-        k = (unsigned int)(hint[s] & 3u) << 30 | (k >> 2);
+    if (hint)  // learned hint first: states on the same solver path / of the same
+               // predicted phase count share wavefronts
+        k = (unsigned int)(hint[s] & 15u) << 28 | (k >> 4);
     keys[s] = k;
     idx[s] = static_cast<unsigned int>(s);
 }

@@ -420,7 +420,8 @@ __global__ void mlpFeatureKernel(int64_t n, const double *__restrict__ P,
 }
 
 // Flash hints from predictions: 1 = confident single phase, 2 = confident
-// two or three phases (log-K seed), 0 = no hint (the plain flash).
+// two or three phases (log-K seed), 0 = no hint (the plain flash); bits 2-3 =
+// most probable phase count - 1 (ordering only).
 template <int NC>
 __global__ void mlpHintKernel(int64_t n, const float *__restrict__ prob,
                               const float *__restrict__ logk, float threshold, int32_t modes,
@@ -435,6 +436,11 @@ __global__ void mlpHintKernel(int64_t n, const float *__restrict__ prob,
         h = NN_HINT_SINGLE_PHASE;
     else if ((p1 >= threshold || p2 >= threshold) && (modes & NN_HINT_LOG_K))
         h = NN_HINT_LOG_K;
+    if (modes & NN_HINT_ORDER)
+    {
+        const int cls = (p1 > p0 && p1 >= p2) ? 1 : ((p2 > p0 && p2 > p1) ? 2 : 0);
+        h = uint8_t(h | (cls << 2));
+    }
     hint[s] = h;
     for (int c = 0; c < NC; ++c) seedLogK[s * NC + c] = double(logk[s * NC + c]);
 }

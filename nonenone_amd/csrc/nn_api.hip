@@ -88,6 +88,7 @@ int dispatchNc(int nc, F &&f)
 {
     switch (nc)
     {
+    case 2: return f(std::integral_constant<int, 2>{});
     case 3: return f(std::integral_constant<int, 3>{});
     case 4: return f(std::integral_constant<int, 4>{});
     case 5: return f(std::integral_constant<int, 5>{});
@@ -95,7 +96,7 @@ int dispatchNc(int nc, F &&f)
     case 8: return f(std::integral_constant<int, 8>{});
     default:
         return fail(NN_E_UNSUPPORTED,
-                    "component count not compiled into this library (3, 4, 5, 6, 8)");
+                    "component count not compiled into this library (2, 3, 4, 5, 6, 8)");
     }
 }
 
@@ -417,10 +418,6 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
         return fail(NN_E_INVALID_ARGUMENT, "invalid cubic EOS constants");
     if (fluid->thermo_model == NN_MODEL_CPA)
     {
-        const FluidParams probe = toParams(*fluid);
-        if (cpaScheme(probe) == kCpaGeneral)
-            return fail(NN_E_UNSUPPORTED,
-                        "CPA with several donor-bearing components (general site solver) is not built");
         if (fluid->cpa_physical_term != 0 && fluid->cpa_physical_term != 1)
             return fail(NN_E_INVALID_ARGUMENT, "cpa_physical_term must be 0 (SRK) or 1 (PR)");
         for (int i = 0; i < fluid->nc; ++i)

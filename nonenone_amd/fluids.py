@@ -179,6 +179,33 @@ def cpa_h2o_co2_nc10() -> FluidDesc:
     return d
 
 
+def cpa_water_methanol() -> FluidDesc:
+    """sCPA water/methanol of test/src/benchmark/cpa_phase_benchmark.cpp:69-91
+    (SRK physical term, 4C water + 2B methanol: the general multi-donor site
+    solver), the state of the reference's published CPA micro-benchmark."""
+    d = FluidDesc()
+    d.nc = 2
+    d.thermo_model = NN_MODEL_CPA
+    d.eos_type = 1
+    d.water_component = 0
+    d.omega_a, d.omega_b, d.m1, d.m2 = 0.42748, 0.08664, 1.0, 0.0
+    tc, pc, vc = (647.3, 512.6), (22.0483e6, 8.0959e6), (5.60e-5, 1.18e-4)
+    om, mw = (0.344, 0.559), (0.018015, 0.032042)
+    for i in range(2):
+        d.tc[i], d.pc[i], d.vc[i], d.acentric[i], d.molar_mass[i] = tc[i], pc[i], vc[i], om[i], mw[i]
+    d.kij[0 * NN_MAX_NC + 1] = d.kij[1 * NN_MAX_NC + 0] = -0.09
+    d.cpa_physical_term = 0
+    a0, b, c1 = (0.12277, 0.40531), (1.4515e-5, 3.0978e-5), (0.67359, 0.43102)
+    eps, beta, sites = (16655.0, 24591.0), (0.0692, 0.0161), (2, 1)
+    for i in range(2):
+        d.cpa_a0[i], d.cpa_b[i], d.cpa_c1[i] = a0[i], b[i], c1[i]
+        d.cpa_epsilon[i], d.cpa_beta[i] = eps[i], beta[i]
+        d.cpa_donors[i] = d.cpa_acceptors[i] = sites[i]
+    d.cpa_max_iterations, d.cpa_tolerance, d.cpa_damping = 2000, 1e-12, 0.2
+    d.relperm[:] = BENCHMARK_RELPERM
+    return d
+
+
 def n_primary(nc: int, has_well: bool) -> int:
     """Indices::numPrimaryVariables: p, 3(nc-1) compositions, 3 saturations [, BHP]."""
     return 3 * nc + 1 + (1 if has_well else 0)

@@ -6,20 +6,24 @@ results that this library's own GPU flash produces, with the library's own HIP
 forward/backward/Adam kernels (FP32), and runs BF16 inference.  Its prediction
 becomes a per-state hint of ``ThreePhaseFlash.flash_seeded``:
 
+* NN_HINT_ORDER: the predicted phase count orders the batch (states of the
+  same predicted count share wavefronts); the solver path is the reference's,
+  so results are bit-identical to the unhinted flash — the mode that can never
+  change a label;
 * NN_HINT_SINGLE_PHASE (p(1 phase) >= threshold): the flash skips its first
   gas-oil two-phase start and goes straight to the single-phase candidate;
 * NN_HINT_LOG_K (p(2) or p(3) >= threshold): that start uses exp(predicted ln K)
   instead of Wilson's K;
 * otherwise the reference path runs unchanged.
 
-The active-set equilibration's stability tests, the three-phase flash and the
-whole recovery chain still run as in the reference and judge the hinted start.
-That judgement is not complete: the reference's stability seeds can miss a
-split that its Wilson-started two-phase flash finds, so a skipped start can
-change a label.  ``calibrate`` therefore lets the full method judge the
-predictor on a validation sample and keeps the lowest confidence threshold with
-zero label and status changes there; ``evaluate`` measures what the hints
-change on fresh states: labels, values and flash time.
+With the two path-changing hints, the active-set equilibration's stability
+tests, the three-phase flash and the recovery chain still run and judge the
+hinted start, but that judgement is not complete: the reference's stability
+seeds can miss a split that its Wilson-started two-phase flash finds, so a
+skipped start can change a label (measured: about 3 in 10^4 hinted states).
+``calibrate`` lets the full method judge the predictor on a validation sample
+and keeps the lowest confidence threshold with zero changes there;
+``evaluate`` measures what hints change on fresh states.
 """
 from __future__ import annotations
 
@@ -125,8 +129,7 @@ class PhaseInitModel:
                                            _stream()), "nn_mlp_predict")
         return prob, logk
 
-    def hints(self, prob, logk, threshold: float = 0.95,
-              modes: int = _abi.NN_HINT_SINGLE_PHASE | _abi.NN_HINT_LOG_K):
+    def hints(self, prob, logk, threshold: float = 0.95, modes: int = _abi.NN_HINT_ORDER):
         n = prob.shape[0]
         hint = torch.empty(n, dtype=torch.uint8, device=prob.device)
         seed = torch.empty(n, self.nc, dtype=torch.float64, device=prob.device)
@@ -254,7 +257,7 @@ def calibrate(flash: ThreePhaseFlash, model: PhaseInitModel, P, T, z, modes: int
 
 
 def evaluate(flash: ThreePhaseFlash, model: PhaseInitModel, P, T, z, threshold: float = 0.95,
-             modes: int = _abi.NN_HINT_SINGLE_PHASE | _abi.NN_HINT_LOG_K, repeats: int = 1):
+             modes: int = _abi.NN_HINT_ORDER, repeats: int = 1):
     """Plain flash vs hinted flash on the same states: labels, values, times, hint counts."""
     P, T, z = _dev(P, torch.float64), _dev(T, torch.float64), _dev(z, torch.float64)
     n = P.shape[0]
@@ -284,11 +287,18 @@ def evaluate(flash: ThreePhaseFlash, model: PhaseInitModel, P, T, z, threshold: 
 
     t_seeded = timed(hinted)
     hint = state["hint"]
+    path = hint & 3
     cmp = compare(plain, seeded)
+    identical = all(torch.equal(getattr(plain, f), getattr(seeded, f))
+                    for f in ("presence", "status", "converged", "iterations", "beta",
+                              "saturation", "x", "Z", "rho_m", "k_go", "k_wo"))
     cmp.update(
         n=n, plain_s=t_plain, seeded_s=t_seeded, speedup=t_plain / t_seeded,
-        hinted_single=int((hint == _abi.NN_HINT_SINGLE_PHASE).sum().item()),
-        hinted_log_k=int((hint == _abi.NN_HINT_LOG_K).sum().item()),
+        bit_identical=identical,
+        hinted_single=int((path == _abi.NN_HINT_SINGLE_PHASE).sum().item()),
+        hinted_log_k=int((path == _abi.NN_HINT_LOG_K).sum().item()),
+        predicted_counts=torch.bincount((hint >> 2).to(torch.int64) & 3,
+                                        minlength=3).tolist(),
         iterations_plain=int(plain.iterations.to(torch.int64).sum().item()),
         iterations_seeded=int(seeded.iterations.to(torch.int64).sum().item()))
     return cmp, plain, seeded

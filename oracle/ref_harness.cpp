@@ -491,6 +491,7 @@ int dispatchNc(int nc, F &&f)
 {
     switch (nc)
     {
+    case 2: return f(std::integral_constant<int, 2>{});
     case 3: return f(std::integral_constant<int, 3>{});
     case 4: return f(std::integral_constant<int, 4>{});
     case 6: return f(std::integral_constant<int, 6>{});

@@ -63,6 +63,23 @@ def assembly_fixture():
           np.bincount(upd_ps[:, 0].astype(int), minlength=8))
 
 
+def cpa_general_fixture():
+    """Water/methanol sCPA (general multi-donor site solver): single-phase EOS
+    results of the reference for seeded (P, T, x, role)."""
+    fluid = fluids.cpa_water_methanol()
+    ref = refharness.Reference(fluid, FlashOptions.defaults())
+    rng = np.random.Generator(np.random.PCG64(17))
+    n = 512
+    P = rng.uniform(1e5, 5e6, n)
+    T = rng.uniform(300.0, 450.0, n)
+    x0 = rng.uniform(0.02, 0.98, n)
+    x = np.stack([x0, 1.0 - x0], axis=1)
+    role = rng.integers(0, 2, n).astype(np.int32)
+    Z, lnphi, fug, st = ref.eos_phase(P, T, x, role)
+    np.savez_compressed(HERE / "eos_cpa_water_methanol.npz", P=P, T=T, x=x, role=role, Z=Z,
+                        lnphi=lnphi, fug=fug, status=st)
+
+
 def main():
     """Writes the fixtures that are missing (all of them with --all)."""
     if not refharness.available():
@@ -70,6 +87,8 @@ def main():
     everything = "--all" in sys.argv
     if everything or not (HERE / "assembly_nc3_pr_12x8.npz").exists():
         assembly_fixture()
+    if everything or not (HERE / "eos_cpa_water_methanol.npz").exists():
+        cpa_general_fixture()
     for name, (make, seed) in CASES.items():
         if not everything and (HERE / f"{name}.npz").exists():
             continue

@@ -299,3 +299,75 @@ def test_gpu_cooperative_pass_matches_golden(name, monkeypatch):
           f"{len(unexplained)} other {unexplained}")
     assert n_status == 0 and n_label == 0
     assert not unexplained
+
+
+# Published figure of the reference's CPA micro-benchmark: water/methanol sCPA
+# (general multi-donor site solver), liquid root at 389.375 K, 0.233183 MPa,
+# x = [0.185, 0.815]: Z = 0.002920191895 (case/five_component_eos_tuned_compare/
+# RUN_REPORT.md:104, state of test/src/benchmark/cpa_phase_benchmark.cpp:144-147,
+# printed to 12 decimals).
+CPA_BENCH_STATE = (233183.46666657156, 389.375, (0.185, 0.815))
+CPA_BENCH_Z = 0.002920191895
+
+
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+def test_reference_build_reproduces_published_cpa_compressibility():
+    P, T, x = CPA_BENCH_STATE
+    ref = refharness.Reference(fluids.cpa_water_methanol(), FlashOptions.defaults())
+    Z, _, _, st = ref.eos_phase(np.array([P]), np.array([T]), np.array([x]), np.array([0], np.int32))
+    assert st[0] == 0 and abs(Z[0] - CPA_BENCH_Z) <= 5e-13
+
+
+@pytest.mark.gpu
+def test_gpu_cpa_general_site_solver_matches_published_and_reference():
+    """E6 general multi-donor association (water 4C + methanol 2B): the published
+    benchmark compressibility, and 512 seeded states against the reference's
+    outputs (golden fixture written by tests/golden/make_golden.py)."""
+    fluid = fluids.cpa_water_methanol()
+    fl = _flash_gpu(fluid)
+    P, T, x = CPA_BENCH_STATE
+    Z, _, _, st = (t.cpu().numpy() for t in fl.eos_phase([P], [T], [x], [0]))
+    assert st[0] == 0 and abs(Z[0] - CPA_BENCH_Z) <= 5e-13, Z[0]
+    g = dict(np.load(GOLDEN / "eos_cpa_water_methanol.npz"))
+    Z, _, fug, st = (t.cpu().numpy() for t in fl.eos_phase(g["P"], g["T"], g["x"], g["role"]))
+    assert np.array_equal(st, g["status"])
+    ok = g["status"] == 0
+    dz = np.abs(Z[ok] - g["Z"][ok]) / np.abs(g["Z"][ok])
+    df = np.abs(fug[ok] - g["fug"][ok]) / np.maximum(np.abs(g["fug"][ok]), 1e-300)
+    print(f"general CPA: {int(ok.sum())} states, max rel Z {dz.max():.2e}, fugacity {df.max():.2e}")
+    assert ok.sum() > len(ok) // 2 and dz.max() <= 1e-10 and df.max() <= 1e-10
+
+
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+def test_cpa_general_fixture_matches_reference_build():
+    g = dict(np.load(GOLDEN / "eos_cpa_water_methanol.npz"))
+    ref = refharness.Reference(fluids.cpa_water_methanol(), FlashOptions.defaults())
+    Z, _, fug, st = ref.eos_phase(g["P"], g["T"], g["x"], g["role"])
+    assert np.array_equal(st, g["status"]) and np.array_equal(Z, g["Z"])
+
+
+def test_host_build_of_cpa_matches_general_fixture(tmp_path):
+    """The CPA code of this library compiled for the host (tests/cpa_host_check.cpp)
+    reproduces the reference's general-site-solver fixture; a CPU check of the
+    same source the GPU runs."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    exe = tmp_path / "cpa_host_check"
+    subprocess.run([hipcc, "-std=c++17", "-O1", "-ffp-contract=off", "-x", "hip",
+                    "--cuda-host-only", str(Path(__file__).parent / "cpa_host_check.cpp"),
+                    "-o", str(exe)], check=True, capture_output=True)
+    g = dict(np.load(GOLDEN / "eos_cpa_water_methanol.npz"))
+    fd = tmp_path / "fluid.bin"
+    fd.write_bytes(bytes(fluids.cpa_water_methanol()))
+    st = tmp_path / "states.bin"
+    with open(st, "wb") as f:
+        np.array([len(g["P"])], np.int64).tofile(f)
+        for k in ("P", "T", "x"):
+            np.ascontiguousarray(g[k], np.float64).tofile(f)
+        np.ascontiguousarray(g["role"], np.int32).tofile(f)
+    out = subprocess.run([str(exe), str(fd), str(st)], capture_output=True, text=True, check=True)
+    rows = np.array([[float(v) for v in line.split()] for line in out.stdout.splitlines()])
+    assert np.array_equal(rows[:, 0].astype(np.int32), g["status"])
+    dz = np.abs(rows[:, 1] - g["Z"]) / np.abs(g["Z"])
+    assert dz.max() <= 1e-12, dz.max()
