@@ -37,9 +37,10 @@ Other workloads (same JSON contract, for the figures quoted beside the headline)
                         with launch shapes picked by the autotuner on an 8192-state
                         sample of each sub-batch;
                         also times the plain one-kernel form on the same states.
-    --workload serving  64 ensemble members (nc=3/5/6/8, PR/SW/PR-CPA, 1k-64k cells each)
+    --workload serving  64 ensemble members (nc=3/5/6/8, PR/SW/PR-CPA, 1k-32k cells each)
                         submit 256 Newton-step requests with Poisson arrivals (200/s);
-                        continuous batching on 8 stream slots vs the plain
+                        continuous batching (phase updates of members sharing a
+                        fluid merged into one launch) vs the plain
                         one-request-at-a-time form on the same trace.
     --workload cpa_phase  the reference's published CPA micro-benchmark state
                         (water/methanol sCPA, general site solver) as 2^20 GPU
@@ -494,24 +495,28 @@ def serving_workload(args) -> dict:
     t0 = time.time()
     for i in range(args.members):
         kind = SERVING_KINDS[i % len(SERVING_KINDS)]
-        hi = args.max_member_cells if kind != "nc3-CPA" else min(args.max_member_cells, 8192)
+        # the costlier fluids get smaller grids (PR-CPA ~30x, SW / nc=8 ~5x the PR cost)
+        cap = 4096 if "CPA" in kind else (8192 if ("SW" in kind or "nc8" in kind) else None)
+        hi = args.max_member_cells if cap is None else min(args.max_member_cells, cap)
         cells = int(np.exp(rng.uniform(np.log(1000), np.log(hi))))
         nx = 50
         ny = max(1, (cells + nx - 1) // nx)
+        t1 = time.time()
         members.append(Member(kind, nx, ny, seed=7000 + i))
+        log(f"member {i} ({kind}, {nx * ny} cells) set up in {time.time() - t1:.1f}s")
     total_cells = sum(m.ncell for m in members)
     log(f"{len(members)} members, {total_cells} cells in total ({time.time() - t0:.1f}s set-up)")
     arrivals = poisson_arrivals(args.requests, len(members), rate=args.rate, seed=11)
     budget = int(args.budget_gb * (1 << 30))
     # warm-up: every member once (first launches allocate each context's queues)
-    run_trace(Server(args.slots, budget), members, [(0.0, i) for i in range(len(members))])
+    run_trace(Server(args.batch_steps, budget), members, [(0.0, i) for i in range(len(members))])
     plain = run_trace(Server(1, budget), members, arrivals)
     log(f"plain form: {plain.cells} cell-steps in {plain.makespan_s:.3f}s, "
         f"p50 {np.percentile(plain.latencies_ms, 50):.1f} ms, "
         f"p99 {np.percentile(plain.latencies_ms, 99):.1f} ms")
-    cont = run_trace(Server(args.slots, budget), members, arrivals)
+    cont = run_trace(Server(args.batch_steps, budget), members, arrivals)
     same = dict(plain.checksums) == dict(cont.checksums)
-    log(f"continuous batching ({args.slots} slots): {cont.cells} cell-steps in "
+    log(f"continuous batching (<= {args.batch_steps} steps per batch): {cont.cells} cell-steps in "
         f"{cont.makespan_s:.3f}s, p50 {np.percentile(cont.latencies_ms, 50):.1f} ms, "
         f"p99 {np.percentile(cont.latencies_ms, 99):.1f} ms; stats {cont.stats}; "
         f"results identical to the plain form: {same}")
@@ -523,10 +528,12 @@ def serving_workload(args) -> dict:
         "ms_per_step": cont.makespan_s * 1e3 / len(arrivals), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"serving: {len(members)} members ({', '.join(SERVING_KINDS)}; "
-                               f"1k-{args.max_member_cells} cells each, CPA members <= 8k), "
+                               f"1k-{args.max_member_cells} cells each; PR-CPA <= 4k, SW and "
+                               f"nc=8 <= 8k), "
                                f"{len(arrivals)} Newton-step requests (phase update + residual "
                                f"+ BSR Jacobian), Poisson arrivals at {args.rate}/s, "
-                               f"{args.slots} stream slots, {args.budget_gb} GB output budget",
+                               f"continuous batching (<= {args.batch_steps} steps per batch, "
+                               f"phase updates merged per fluid), {args.budget_gb} GB budget",
                    "members": len(members), "member_cells": total_cells},
         "p50_ms": float(np.percentile(cont.latencies_ms, 50)),
         "p99_ms": float(np.percentile(cont.latencies_ms, 99)),
@@ -554,10 +561,10 @@ def main() -> int:
                     default="assembly")
     ap.add_argument("--tune-sample", type=int, default=8192)
     ap.add_argument("--members", type=int, default=64)
-    ap.add_argument("--max-member-cells", type=int, default=65536)
+    ap.add_argument("--max-member-cells", type=int, default=32768)
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--rate", type=float, default=200.0)
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--batch-steps", type=int, default=64)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--states", type=int, default=1 << 20)
     ap.add_argument("--cpu-sample", type=int, default=2048)

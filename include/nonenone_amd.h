@@ -333,19 +333,28 @@ int nn_assembly_info(const nn_assembly *a, nn_context **ctx, int64_t *ncell,
  * Ensemble members keep their state on the device (primary + phase-state
  * vectors: each cell's K-values and phase hysteresis) and submit steps: the
  * phase update (nn_phase_update_batch) then residual + BSR Jacobian
- * (nn_assembly_evaluate) of the member's grid.  A worker thread admits pending
- * steps in submission order as soon as a stream slot is free, the member has no
- * step in flight, and the step's output buffers fit the memory budget; admitted
- * steps run concurrently on the server's streams (one per slot).  Steps of one
- * member run one at a time in submission order, so results do not depend on the
- * mix.  Each member needs its own context: a context's work queue serves
- * one launch at a time.  Output buffers belong to the step until
- * nn_server_release.  max_inflight = 1 gives the plain one-request-at-a-time form.
+ * (nn_assembly_evaluate) of the member's grid.  A worker thread forms batches
+ * continuously: whenever fewer than two batches are in flight it takes pending
+ * steps in submission order (at most max_batch_steps, members with no step in
+ * flight, output buffers within the memory budget), merges the cells of steps
+ * with the same fluid_group into ONE phase-update launch (gather into a staging
+ * buffer, update, scatter back), then runs each step's assembly.  Steps that
+ * arrive while a batch runs join the next one.  Everything is issued on one
+ * stream (the one given at creation, NULL = default): kernels whose lanes need
+ * tens of KB of scratch could not be given scratch on additional hardware
+ * queues, and merged launches fill the GPU without them.  The phase update is
+ * per cell, so results are the same alone and in any mix.  Steps of one member
+ * run in submission order.  Output buffers belong to the step until
+ * nn_server_release.  max_batch_steps = 1 gives the plain one-request-at-a-time
+ * form.  fluid_group: steps whose members share fluid and flash options may
+ * carry the same group id >= 0 (the first step's context runs the merged
+ * update); -1 never merges.
  */
 typedef struct nn_server nn_server;
 typedef struct nn_step_request {
     nn_assembly *assembly;       /* member's grid; its context runs the phase update */
     int64_t member;              /* member id (serialises that member's steps)       */
+    int64_t fluid_group;         /* merge key of the phase update, -1 = alone        */
     int32_t has_well_unknown;
     int32_t want_bsr;            /* 0: residual only                                 */
     double dt, fugacity_scaling;
@@ -359,6 +368,7 @@ typedef struct nn_step_result {
     int32_t np;
     int64_t ncell, nnzb;
     double submit_s, start_s, end_s;  /* server clock: submission, launch, completion seen */
+    int64_t batch;               /* id of the batch the step ran in                  */
     double *d_residual;          /* ncell x np                                       */
     double *d_bsr;               /* nnzb x np x np (NULL without want_bsr)           */
     int32_t *d_update_status;    /* ncell NN_STATUS_* of the phase update            */
@@ -366,11 +376,13 @@ typedef struct nn_step_result {
     uint8_t *d_update_flags;     /* ncell                                            */
 } nn_step_result;
 typedef struct nn_server_stats {
-    int64_t completed, pending;
-    int64_t inflight, peak_inflight;
+    int64_t completed, pending, batches;
+    int64_t inflight, peak_inflight;   /* steps in flight                          */
+    int64_t peak_batch_steps, peak_merged_cells;
     int64_t memory_in_use, peak_memory_in_use, cached_bytes;
 } nn_server_stats;
-int nn_server_create(int32_t max_inflight, int64_t memory_budget_bytes, nn_server **out);
+int nn_server_create(int32_t max_batch_steps, int64_t memory_budget_bytes, void *stream,
+                     nn_server **out);
 int nn_server_destroy(nn_server *s);   /* finishes every submitted step first */
 int nn_server_submit_step(nn_server *s, const nn_step_request *req, int64_t *ticket);
 int nn_server_wait(nn_server *s, int64_t ticket, nn_step_result *res);  /* blocks */

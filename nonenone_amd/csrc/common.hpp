@@ -40,6 +40,10 @@ enum : int
     // Internal: the phase update recorded its stability test's seed-trial inputs
     // for the lockstep trial pass and stops; never reported to callers.
     kErrPending = 65,
+    // Internal: the replayed phase update found a missing phase unstable and
+    // stops before its unrestricted re-flash, which a later pass runs for all
+    // such cells together; never reported to callers.
+    kErrReflash = 66,
 };
 
 // Stability-test record of one cell (phase update, three passes): the
@@ -51,6 +55,7 @@ enum : int
     kStabNone = 0,
     kStabRecord = 1,
     kStabReplay = 2,
+    kStabReplayHoldReflash = 3,  // replay, and hand the re-flash to a later pass
 };
 
 template <int NC>
@@ -71,6 +76,7 @@ enum : int
     kItemDone = 0,
     kItemDeferred = 1,
     kItemPending = 2,
+    kItemReflash = 3,
 };
 
 // Execution modes of the flash code.

@@ -60,6 +60,7 @@ struct CpaAssoc
 // -2 - k for a single donor-bearing component k, or -1 for none, or
 // kCpaGeneral when several components carry donor sites.
 constexpr int kCpaGeneral = -1000;
+constexpr int kCpaGeneralMaxNc = 4;
 
 __host__ __device__ inline int cpaScheme(const FluidParams &f)
 {
@@ -228,9 +229,13 @@ __host__ __device__ bool cpaSolveDense(S (&a)[M][M], S (&b)[M], int n, S (&x)[M]
 // residuals X (1 + sum C X') - 1 with a halving line search that keeps
 // 0 < X <= 1 and lowers the residual; damped fixed point from X = 1 when
 // Newton fails; thermo error when that does not converge either.
+// Out of line: only fluids with several donor-bearing components take this
+// path, and its dense Newton would otherwise be inlined into every EOS call of
+// the flash, phase-update and assembly kernels of all fluids.
 template <int NC, class S>
-__host__ __device__ void cpaGeneralSites(const FluidParams &f, const CpaTemp<NC> &c, const S &rho,
-                                         const S *x, CpaAssoc<NC, S> &st, Err &err)
+__host__ __device__ __noinline__ void cpaGeneralSites(const FluidParams &f, const CpaTemp<NC> &c,
+                                                      const S &rho, const S *x,
+                                                      CpaAssoc<NC, S> &st, Err &err)
 {
     constexpr int M = 2 * NC;
     int dU[NC], aU[NC];
@@ -465,7 +470,13 @@ __host__ __device__ void cpaAssociation(const FluidParams &f, const CpaTemp<NC> 
     }
     else if (scheme == kCpaGeneral)
     {
-        cpaGeneralSites<NC, S>(f, c, rho, x, st, err);
+        // Built for up to kCpaGeneralMaxNc components: beyond that the dense
+        // AD Jacobian of the site Newton outgrows a lane's stack (contexts
+        // reject such fluids).
+        if constexpr (NC <= kCpaGeneralMaxNc)
+            cpaGeneralSites<NC, S>(f, c, rho, x, st, err);
+        else
+            err.raise(kErrInvalid);
         if (err.bad())
             return;
     }

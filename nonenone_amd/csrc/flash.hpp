@@ -243,6 +243,7 @@ public:
     {
         if (!(P > 0.0) || !finite(P) || !(T > 0.0) || !finite(T))
         {
+            NN_TRACE("invalid P/T %.17g %.17g\n", P, T);
             err.raise(kErrInvalid);
             return false;
         }
@@ -256,6 +257,7 @@ public:
         {
             if (!finite(x[c]))
             {
+                NN_TRACE("invalid composition entry %d = %.17g\n", c, x[c]);
                 err.raise(kErrInvalid);
                 return;
             }
@@ -264,6 +266,7 @@ public:
         }
         if (!(sum > 0.0))
         {
+            NN_TRACE("invalid composition sum %.17g\n", sum);
             err.raise(kErrInvalid);
             return;
         }
@@ -2139,6 +2142,7 @@ public:
             return result;
         if (active == 0)
         {
+            NN_TRACE("stability with no active phase\n");
             err.raise(kErrInvalid);
             return result;
         }
@@ -3530,6 +3534,7 @@ __host__ __device__ FlashResult<NC> Flash<NC, Mode>::flashRestrictedImpl(double 
     normalize(z);
     if (allowed == 0)
     {
+        NN_TRACE("restricted flash with nothing allowed\n");
         err.raise(kErrInvalid);
         return fail;
     }
@@ -3544,6 +3549,7 @@ __host__ __device__ FlashResult<NC> Flash<NC, Mode>::flashRestrictedImpl(double 
         // the public all-phase request is routed to flash() by the kernel, so
         // it is rejected here: calling flash() from inside it would make the
         // device call graph recursive and its stack size unbounded.
+        NN_TRACE("restricted flash with all phases allowed\n");
         err.raise(kErrInvalid);
         return fail;
     }

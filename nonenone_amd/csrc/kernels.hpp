@@ -387,15 +387,39 @@ __global__ void __launch_bounds__(kBlock) stabilityTrialKernel(
 // stability test replayed from the lockstep trials' results.
 template <int NC, bool Well>
 __global__ void __launch_bounds__(kBlock) phaseUpdateReplayKernel(
-    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
     int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
     unsigned int *__restrict__ list, const unsigned int *__restrict__ pendingList,
     StabRecord<NC> *__restrict__ records)
 {
     const int64_t count = int64_t(q->pending);
+    unsigned int *reflashList = const_cast<unsigned int *>(pendingList) + n;
     drainCounter(q, &q->replayHead, count, [&](int64_t i) {
         const int64_t s = int64_t(pendingList[i]);
+        const int r = phaseUpdateCell<NC, Well, kModeDefer>(
+            *fp, *op, T[s], primary, phaseState, s, status, flags, kStabReplayHoldReflash, records);
+        if (r == kItemDeferred)
+            deferItem(q, list, s);
+        else if (r == kItemReflash)
+            reflashList[atomicAdd(&q->reflash, 1u)] = static_cast<unsigned int>(s);
+    });
+}
+
+// Re-flash pass: the replayed cells whose missing phase turned out unstable,
+// together, so that the wavefronts of this costly unrestricted flash are full.
+template <int NC, bool Well>
+__global__ void __launch_bounds__(kBlock) phaseUpdateReflashKernel(
+    const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
+    const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
+    int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
+    unsigned int *__restrict__ list, const unsigned int *__restrict__ pendingList,
+    StabRecord<NC> *__restrict__ records)
+{
+    const int64_t count = int64_t(q->reflash);
+    const unsigned int *reflashList = pendingList + n;
+    drainCounter(q, &q->reflashHead, count, [&](int64_t i) {
+        const int64_t s = int64_t(reflashList[i]);
         const int r = phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary, phaseState, s,
                                                             status, flags, kStabReplay, records);
         if (r == kItemDeferred)
@@ -752,10 +776,18 @@ hipError_t Launch<NC>::phaseUpdate(bool well, unsigned grid, unsigned coopGrid, 
             return e;
         if (well)
             phaseUpdateReplayKernel<NC, true><<<grid, kBlock, 0, st>>>(
-                fp, op, T, primary, phaseState, status, flags, q, list, pendingList, recs);
+                fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs);
         else
             phaseUpdateReplayKernel<NC, false><<<grid, kBlock, 0, st>>>(
-                fp, op, T, primary, phaseState, status, flags, q, list, pendingList, recs);
+                fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        if (well)
+            phaseUpdateReflashKernel<NC, true><<<grid, kBlock, 0, st>>>(
+                fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs);
+        else
+            phaseUpdateReflashKernel<NC, false><<<grid, kBlock, 0, st>>>(
+                fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
     }

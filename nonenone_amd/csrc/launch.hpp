@@ -41,7 +41,9 @@ struct WorkQueue
     unsigned int claim;     // 0: lanes claim single items; 1: wavefronts claim 64-item tiles
     unsigned int pending;   // items recorded for the lockstep stability trials (list length)
     unsigned int trialHead; // next pending item of the lockstep trial pass
-    unsigned int replayHead;// next pending item of the finishing pass
+    unsigned int replayHead;// next pending item of the replay pass
+    unsigned int reflash;   // items whose re-flash the replay pass held back (list length)
+    unsigned int reflashHead; // next item of the re-flash pass
 };
 
 // Per-state hints of the learned initialiser (both may be null): hint[s] is
@@ -76,8 +78,10 @@ struct Launch
                                 double *trial, double *incipient, int32_t *status);
     // `pendingList` / `records` (may be null: no lockstep trials): cells whose
     // direct stability test is recorded by the lane-per-cell pass, run by the
-    // lockstep trial pass and finished by the replay pass, all before the
-    // cooperative pass.  `records` holds n StabRecord<NC>.
+    // lockstep trial pass and finished by the replay pass — whose cells that
+    // must re-flash go to a last pass of their own (`pendingList + n`) — all
+    // before the cooperative pass.  `pendingList` holds 2n entries, `records`
+    // n StabRecord<NC>.
     static hipError_t phaseUpdate(bool well, unsigned grid, unsigned coopGrid, hipStream_t st,
                                   const FluidParams *fp, const FlashOptions *op, int64_t n,
                                   const double *T, double *primary, double *phaseState,

@@ -113,6 +113,8 @@ __global__ void resetQueueKernel(WorkQueue *q, unsigned int deferAll, unsigned i
     q->pending = 0u;
     q->trialHead = 0u;
     q->replayHead = 0u;
+    q->reflash = 0u;
+    q->reflashHead = 0u;
 }
 
 void releaseEvents(nn_context *ctx)
@@ -201,7 +203,7 @@ int ensureRecords(nn_context *ctx, int64_t n, size_t bytes)
     ctx->dPending = nullptr;
     ctx->dRecords = nullptr;
     ctx->recordCap = 0;
-    if (hipMalloc(&ctx->dPending, sizeof(unsigned int) * size_t(n)) != hipSuccess ||
+    if (hipMalloc(&ctx->dPending, 2 * sizeof(unsigned int) * size_t(n)) != hipSuccess ||
         hipMalloc(&ctx->dRecords, bytes * size_t(n)) != hipSuccess)
         return fail(NN_E_OUT_OF_MEMORY, "device allocation of the stability records failed");
     ctx->recordCap = n;
@@ -418,6 +420,9 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
         return fail(NN_E_INVALID_ARGUMENT, "invalid cubic EOS constants");
     if (fluid->thermo_model == NN_MODEL_CPA)
     {
+        if (cpaScheme(toParams(*fluid)) == kCpaGeneral && fluid->nc > kCpaGeneralMaxNc)
+            return fail(NN_E_UNSUPPORTED,
+                        "CPA with several donor-bearing components is built for nc <= 4");
         if (fluid->cpa_physical_term != 0 && fluid->cpa_physical_term != 1)
             return fail(NN_E_INVALID_ARGUMENT, "cpa_physical_term must be 0 (SRK) or 1 (PR)");
         for (int i = 0; i < fluid->nc; ++i)

@@ -426,6 +426,8 @@ struct PhaseUpdater
         if (removed)
         {
             const Result reduced = flash.flashRestricted(P, T, overall, active, comps);
+            NN_TRACE("update: restricted flash active=%d conv=%d pres=%d err=%d\n", (int)active,
+                     (int)reduced.converged, (int)reduced.presence, err.code);
             if (err.bad())
                 return flags;
             if (reduced.converged)
@@ -435,6 +437,8 @@ struct PhaseUpdater
                 {
                     const StabilityResult<NC> st =
                         flash.stabilityTest(P, T, overall, reduced.presence, reduced.x);
+                    NN_TRACE("update: stability of reduced set valid=%d err=%d\n", (int)st.valid,
+                             err.code);
                     if (err.bad())
                         return flags;
                     if (!st.valid)
@@ -463,6 +467,8 @@ struct PhaseUpdater
                 }
             }
             const Result reflashed = flash.flash(P, T, overall);
+            NN_TRACE("update: reflash conv=%d pres=%d err=%d\n", (int)reflashed.converged,
+                     (int)reflashed.presence, err.code);
             if (err.bad())
                 return flags;
             if (reflashed.converged)
@@ -479,7 +485,7 @@ struct PhaseUpdater
         }
         else if (popcount3(active) < 3)
         {
-            flash.stabMode = stabMode;
+            flash.stabMode = stabMode == kStabReplayHoldReflash ? kStabReplay : stabMode;
             flash.stabRec = stabRec;
             const StabilityResult<NC> st = flash.stabilityTest(P, T, overall, active, comps);
             flash.stabMode = kStabNone;
@@ -493,6 +499,11 @@ struct PhaseUpdater
                 flags |= kFlagMissingUnstable;
             if (invalid || missing)
             {
+                if (stabMode == kStabReplayHoldReflash)
+                {
+                    err.raise(kErrReflash);
+                    return flags;
+                }
                 const Result reflashed = flash.flash(P, T, overall);
                 if (err.bad())
                     return flags;
@@ -572,9 +583,11 @@ __host__ __device__ void encodePhaseState(const PhaseStateData<NC> &ps, double *
 
 // One cell of nn_phase_update_batch (natural_petsc_runtime_state.inc:215-221):
 // kItemDone, kItemDeferred (a kModeDefer run handed the cell to the cooperative
-// pass) or kItemPending (kStabRecord: its stability-test inputs are in rec; the
-// lockstep trial pass and a kStabReplay run finish it).  Deferred and pending
-// cells leave primary/phaseState untouched; in kModeCoop only lane 0 writes.
+// pass), kItemPending (kStabRecord: its stability-test inputs are in rec; the
+// lockstep trial pass and a replay finish it) or kItemReflash
+// (kStabReplayHoldReflash: its re-flash is left to a pass over all such cells).
+// Deferred, pending and held cells leave primary/phaseState untouched; in
+// kModeCoop only lane 0 writes.
 template <int NC, bool Well, int Mode = kModePlain>
 __host__ __device__ int phaseUpdateCell(const FluidParams &f, const FlashOptions &o, double T,
                                         double *primary, double *phaseState, int64_t s,
@@ -610,6 +623,8 @@ __host__ __device__ int phaseUpdateCell(const FluidParams &f, const FlashOptions
         return kItemDeferred;
     if (err.code == kErrPending)
         return kItemPending;
+    if (err.code == kErrReflash)
+        return kItemReflash;
     if (!writer)
         return kItemDone;
     if (err.bad())

@@ -2,11 +2,13 @@
 counterpart in the reference, whose runtime assembles one simulation per process).
 
 Host mirror of the nn_server C ABI (continuous batching in the library's native
-worker thread) plus the pieces a driver needs: ``Member`` (a device-resident
-ensemble member: its own context, grid, assembly and state — the phase-state
-vector holds each cell's K-values and hysteresis between requests) and
-``run_trace`` (submit steps at given arrival times, wait, collect latencies and
-bitwise checksums of every result).
+worker thread: steps that arrive while a batch runs join the next batch, and the
+cells of steps whose members share a fluid go through one merged phase-update
+launch) plus the pieces a driver needs: ``Member`` (a device-resident ensemble
+member: its own context, grid, assembly and state — the phase-state vector holds
+each cell's K-values and hysteresis between requests) and ``run_trace`` (submit
+steps at given arrival times, wait, collect latencies and bitwise checksums of
+every result).
 """
 from __future__ import annotations
 
@@ -28,26 +30,26 @@ _P = ctypes.c_void_p
 
 
 class StepRequest(ctypes.Structure):
-    _fields_ = [("assembly", _P), ("member", _I64), ("has_well_unknown", _I32),
+    _fields_ = [("assembly", _P), ("member", _I64), ("fluid_group", _I64), ("has_well_unknown", _I32),
                 ("want_bsr", _I32), ("dt", _D), ("fugacity_scaling", _D), ("d_T", _P),
                 ("d_primary", _P), ("d_phase_state", _P), ("d_accum_prev", _P)]
 
 
 class StepResult(ctypes.Structure):
     _fields_ = [("rc", _I32), ("np", _I32), ("ncell", _I64), ("nnzb", _I64),
-                ("submit_s", _D), ("start_s", _D), ("end_s", _D), ("d_residual", _P),
+                ("submit_s", _D), ("start_s", _D), ("end_s", _D), ("batch", _I64), ("d_residual", _P),
                 ("d_bsr", _P), ("d_update_status", _P), ("d_assembly_status", _P),
                 ("d_update_flags", _P)]
 
 
 class ServerStats(ctypes.Structure):
-    _fields_ = [("completed", _I64), ("pending", _I64), ("inflight", _I64),
-                ("peak_inflight", _I64), ("memory_in_use", _I64),
-                ("peak_memory_in_use", _I64), ("cached_bytes", _I64)]
+    _fields_ = [("completed", _I64), ("pending", _I64), ("batches", _I64), ("inflight", _I64),
+                ("peak_inflight", _I64), ("peak_batch_steps", _I64), ("peak_merged_cells", _I64),
+                ("memory_in_use", _I64), ("peak_memory_in_use", _I64), ("cached_bytes", _I64)]
 
 
 def declare(L) -> None:
-    L.nn_server_create.argtypes = [_I32, _I64, ctypes.POINTER(_P)]
+    L.nn_server_create.argtypes = [_I32, _I64, _P, ctypes.POINTER(_P)]
     L.nn_server_destroy.argtypes = [_P]
     L.nn_server_submit_step.argtypes = [_P, ctypes.POINTER(StepRequest), ctypes.POINTER(_I64)]
     L.nn_server_wait.argtypes = [_P, _I64, ctypes.POINTER(StepResult)]
@@ -87,13 +89,15 @@ def checksum(t: torch.Tensor) -> int:
 
 
 class Server:
-    """nn_server: continuous batching of member steps onto `max_inflight` streams."""
+    """nn_server: continuous batching of member steps, at most `max_batch_steps` per
+    batch, issued on torch's current stream."""
 
-    def __init__(self, max_inflight: int, memory_budget_bytes: int):
+    def __init__(self, max_batch_steps: int, memory_budget_bytes: int):
         self.lib = _abi.lib()
         declare(self.lib)
         h = _P()
-        rc = self.lib.nn_server_create(int(max_inflight), int(memory_budget_bytes),
+        stream = _P(torch.cuda.current_stream().cuda_stream)
+        rc = self.lib.nn_server_create(int(max_batch_steps), int(memory_budget_bytes), stream,
                                        ctypes.byref(h))
         self._check(rc, "nn_server_create")
         self.handle = h
@@ -116,8 +120,9 @@ class Server:
 
     def submit(self, member_id: int, m: "Member", dt: float = workloads.DAY,
                want_bsr: bool = True) -> int:
-        r = StepRequest(m.asm.handle, int(member_id), int(m.has_well), int(bool(want_bsr)),
-                        float(dt), 1.0, _ptr(m.T), _ptr(m.prim), _ptr(m.ps), _ptr(m.acc))
+        r = StepRequest(m.asm.handle, int(member_id), int(m.group), int(m.has_well),
+                        int(bool(want_bsr)), float(dt), 1.0, _ptr(m.T), _ptr(m.prim), _ptr(m.ps),
+                        _ptr(m.acc))
         t = _I64()
         self._check(self.lib.nn_server_submit_step(self.handle, ctypes.byref(r), ctypes.byref(t)),
                     "nn_server_submit_step")
@@ -159,6 +164,8 @@ class Member:
 
     def __init__(self, kind: str, nx: int, ny: int, seed: int, has_well: bool = False):
         self.kind = kind
+        # members of one kind share fluid and flash options: their phase updates may merge
+        self.group = list(MEMBER_FLUIDS).index(kind)
         self.fluid = MEMBER_FLUIDS[kind]()
         self.nc = self.fluid.nc
         self.has_well = has_well

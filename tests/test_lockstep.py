@@ -49,6 +49,7 @@ def _run(exe, fluid, T, primary, phase_state, has_well, tmp):
     out = subprocess.run([str(exe), str(fd), str(cells)], capture_output=True, text=True)
     rows = np.array([[int(v) for v in line.split()] for line in out.stdout.splitlines()])
     assert rows.shape == (n, 3), out.stderr
+    print(out.stderr.strip())
     return rows
 
 

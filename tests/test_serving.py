@@ -1,6 +1,7 @@
-"""Serving (continuous batching of member Newton steps): each step's result is the
-same alone and in the mix, memory admission holds, and the server's steps match
-the library's direct calls bit for bit."""
+"""Serving (continuous batching of member Newton steps, merged phase updates of
+members that share a fluid): each step's result is the same alone and in the
+mix, memory admission holds, and the server's steps match the library's direct
+calls bit for bit."""
 from __future__ import annotations
 
 import numpy as np
@@ -18,11 +19,15 @@ def _members():
 def test_gpu_results_are_the_same_alone_and_in_the_mix():
     from nonenone_amd.serving import Server, poisson_arrivals, run_trace
     members = _members()
-    arrivals = poisson_arrivals(18, len(members), rate=2000.0, seed=3)
+    # three steps of every member, all waiting at once: whole batches, and the two
+    # nc3-PR members' phase updates merge; then a Poisson trace of mixed arrivals
+    arrivals = [(0.0, m) for _ in range(3) for m in range(len(members))]
+    arrivals += [(t + 0.05, m) for t, m in poisson_arrivals(12, len(members), 2000.0, seed=3)]
     alone = run_trace(Server(1, 8 << 30), members, arrivals)
-    mixed = run_trace(Server(4, 8 << 30), members, arrivals)
-    assert alone.stats["peak_inflight"] == 1
-    assert mixed.stats["peak_inflight"] > 1, mixed.stats
+    mixed = run_trace(Server(16, 8 << 30), members, arrivals)
+    assert alone.stats["peak_batch_steps"] == 1 and alone.stats["peak_merged_cells"] == 0
+    assert mixed.stats["peak_batch_steps"] > 1, mixed.stats
+    assert mixed.stats["peak_merged_cells"] > 0, mixed.stats   # two nc3-PR members merged
     assert dict(alone.checksums) == dict(mixed.checksums)
     assert alone.nonok == mixed.nonok
 
