@@ -216,7 +216,7 @@ def flash_workload(args) -> dict:
                  z=torch.as_tensor(z, device="cuda"))
         t0 = time.time()
         sample = min(per, args.tune_sample)
-        best, best_ms = fl.autotune(d["P"][:sample], d["T"][:sample], d["z"][:sample], repeats=2)
+        best, best_ms = fl.autotune(d["P"][:sample], d["T"][:sample], d["z"][:sample], repeats=1)
         log(f"{case}: autotune on {sample} states {time.time() - t0:.1f}s -> {best} "
             f"({best_ms:.1f} ms)")
         subs.append(dict(case=case, fl=fl, d=d, host=(P, T, z), sched=best,
@@ -481,8 +481,7 @@ def cpa_phase_workload(args) -> dict:
     }
 
 
-SERVING_KINDS = ("nc3-PR", "nc5-SW", "nc6-PR", "nc8-PR", "nc3-CPA", "nc3-SW", "nc5-PR",
-                 "nc6-SW", "nc8-SW")
+SERVING_KINDS = ("nc3-PR", "nc5-SW", "nc6-PR", "nc3-CPA", "nc3-SW", "nc5-PR", "nc6-SW")
 
 
 def serving_workload(args) -> dict:
@@ -495,8 +494,8 @@ def serving_workload(args) -> dict:
     t0 = time.time()
     for i in range(args.members):
         kind = SERVING_KINDS[i % len(SERVING_KINDS)]
-        # the costlier fluids get smaller grids (PR-CPA ~30x, SW / nc=8 ~5x the PR cost)
-        cap = 4096 if "CPA" in kind else (8192 if ("SW" in kind or "nc8" in kind) else None)
+        # the costlier fluids get smaller grids (PR-CPA ~30x, SW ~5x the PR cost)
+        cap = 4096 if "CPA" in kind else (8192 if "SW" in kind else None)
         hi = args.max_member_cells if cap is None else min(args.max_member_cells, cap)
         cells = int(np.exp(rng.uniform(np.log(1000), np.log(hi))))
         nx = 50
@@ -509,7 +508,9 @@ def serving_workload(args) -> dict:
     arrivals = poisson_arrivals(args.requests, len(members), rate=args.rate, seed=11)
     budget = int(args.budget_gb * (1 << 30))
     # warm-up: every member once (first launches allocate each context's queues)
+    tw = time.time()
     run_trace(Server(args.batch_steps, budget), members, [(0.0, i) for i in range(len(members))])
+    log(f"warm-up (every member once) {time.time() - tw:.1f}s")
     plain = run_trace(Server(1, budget), members, arrivals)
     log(f"plain form: {plain.cells} cell-steps in {plain.makespan_s:.3f}s, "
         f"p50 {np.percentile(plain.latencies_ms, 50):.1f} ms, "
@@ -528,8 +529,7 @@ def serving_workload(args) -> dict:
         "ms_per_step": cont.makespan_s * 1e3 / len(arrivals), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"serving: {len(members)} members ({', '.join(SERVING_KINDS)}; "
-                               f"1k-{args.max_member_cells} cells each; PR-CPA <= 4k, SW and "
-                               f"nc=8 <= 8k), "
+                               f"1k-{args.max_member_cells} cells each; PR-CPA <= 4k, SW <= 8k), "
                                f"{len(arrivals)} Newton-step requests (phase update + residual "
                                f"+ BSR Jacobian), Poisson arrivals at {args.rate}/s, "
                                f"continuous batching (<= {args.batch_steps} steps per batch, "
@@ -559,10 +559,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("assembly", "flash", "learned", "serving", "cpa_phase"),
                     default="assembly")
-    ap.add_argument("--tune-sample", type=int, default=8192)
-    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--tune-sample", type=int, default=4096)
+    ap.add_argument("--members", type=int, default=48)
     ap.add_argument("--max-member-cells", type=int, default=32768)
-    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=192)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--batch-steps", type=int, default=64)
     ap.add_argument("--budget-gb", type=float, default=64.0)

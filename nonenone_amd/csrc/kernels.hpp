@@ -826,6 +826,12 @@ hipError_t Launch<NC>::assemble(bool well, hipStream_t st, const FluidParams *fp
                                 double *residual, double *bsr, uint8_t *decisions,
                                 int32_t *status)
 {
+    if constexpr (NC > kAssemblyMaxNc)
+    {
+        return hipErrorNotSupported;
+    }
+    else
+    {
     const unsigned grid = static_cast<unsigned>((v.ncell + 63) / 64);
     if (well)
         assembleKernel<NC, true><<<grid, 64, 0, st>>>(fp, v, dt, fugScale, primary, phaseState,
@@ -834,6 +840,7 @@ hipError_t Launch<NC>::assemble(bool well, hipStream_t st, const FluidParams *fp
         assembleKernel<NC, false><<<grid, 64, 0, st>>>(fp, v, dt, fugScale, primary, phaseState,
                                                        accumPrev, residual, bsr, decisions, status);
     return hipGetLastError();
+    }
 }
 
 } // namespace nn

@@ -16,6 +16,13 @@ namespace nn
 
 constexpr int kBlock = 64;  // one wavefront per block: divergent per-state control flow
 
+// Largest component count the Natural assembly is built for.  Its forward-mode
+// AD carries (3 NC + 1)-wide dual numbers per lane; at eight components the
+// assembly kernel's private segment is about 85 KB per lane, and its dispatch
+// stopped the queue with HSA_STATUS_ERROR_OUT_OF_RESOURCES (scratch), so larger
+// fluids get flash and phase updates but no assembly.
+constexpr int kAssemblyMaxNc = 6;
+
 struct AssemblyView
 {
     int64_t ncell;

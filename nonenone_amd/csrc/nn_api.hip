@@ -927,6 +927,8 @@ int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
     *out = nullptr;
     if (ctx->host.water < 0)
         return fail(NN_E_INVALID_ARGUMENT, "Natural assembly needs a water component");
+    if (ctx->host.nc > kAssemblyMaxNc)
+        return fail(NN_E_UNSUPPORTED, "Natural assembly is built for at most 6 components");
     // Slots of the diagonal and reciprocal blocks in the cached pattern (setup only).
     std::vector<int64_t> co(ncell + 1), ro(ncell + 1);
     NN_HIP(hipMemcpy(co.data(), d_conn_offsets, sizeof(int64_t) * (ncell + 1), hipMemcpyDeviceToHost));

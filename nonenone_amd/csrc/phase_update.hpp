@@ -466,6 +466,17 @@ struct PhaseUpdater
                     return flags | kFlagStableReduced;
                 }
             }
+            if constexpr (Mode == kModeDefer)
+            {
+                // The lane-per-cell device build returned wrong statuses, flags and
+                // states for cells on this path (restricted flash, stability test,
+                // then the unrestricted re-flash in one lane) at six components,
+                // while the host build of the same code and the wavefront-
+                // cooperative device build were right: such cells go to the
+                // cooperative pass, which redoes the update from the inputs.
+                err.raise(kErrDefer);
+                return flags;
+            }
             const Result reflashed = flash.flash(P, T, overall);
             NN_TRACE("update: reflash conv=%d pres=%d err=%d\n", (int)reflashed.converged,
                      (int)reflashed.presence, err.code);

@@ -94,14 +94,18 @@ def test_gpu_assembly_matches_fixture():
 @pytest.mark.gpu
 @pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("model,has_well", [("PR", False), ("SW", False), ("PR", True),
-                                            ("CPA", False), ("CPA", True)])
+                                            ("CPA", False), ("CPA", True), ("PR6", False),
+                                            ("SW6", False)])
 def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     import torch
     from nonenone_amd.assembly import NaturalAssembly
 
     from nonenone_amd.flash import ThreePhaseFlash
 
-    fluid = fluids.cpa_h2o_co2_nc10() if model == "CPA" else fluids.h2o_co2_nc10(model)
+    if model.endswith("6"):
+        fluid = fluids.six_component(model[:-1])
+    else:
+        fluid = fluids.cpa_h2o_co2_nc10() if model == "CPA" else fluids.h2o_co2_nc10(model)
     ref = refharness.Reference(fluid, FlashOptions.defaults())
     w, hist_p, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 60, 20, 21, ref,
                                                                 has_well=has_well)
@@ -204,3 +208,18 @@ def test_gpu_newton_assembly_step_replays_in_a_hip_graph():
         for k in ("residual", "bsr", "decisions", "status"):
             assert torch.equal(out[k], eager[k]), f"replay {replay}: {k}"
         _ = (prim * 2.0).sum().item()      # eager work between replays
+
+
+@pytest.mark.gpu
+def test_gpu_assembly_refuses_more_than_six_components():
+    """The Natural assembly is built for up to six components (an eight-component
+    fluid gets flash and phase updates only): creation fails with NN_E_UNSUPPORTED
+    before any kernel runs."""
+    from nonenone_amd.assembly import NaturalAssembly
+    from nonenone_amd.flash import ThreePhaseFlash
+
+    _, topo = load_fixture()
+    n = len(topo["conn_offsets"]) - 1
+    ctx = ThreePhaseFlash(fluids.eight_component("PR"))
+    with pytest.raises(RuntimeError, match="at most 6 components"):
+        NaturalAssembly(ctx, topo, np.ones(n), np.full(n, 0.2), np.full(n, 350.0))
