@@ -388,6 +388,10 @@ int nn_server_submit_step(nn_server *s, const nn_step_request *req, int64_t *tic
 int nn_server_wait(nn_server *s, int64_t ticket, nn_step_result *res);  /* blocks */
 int nn_server_release(nn_server *s, int64_t ticket);
 int nn_server_get_stats(nn_server *s, nn_server_stats *out);
+/* hold != 0: issue no new batch (batches already issued run to completion) until
+ * called again with hold = 0; steps submitted meanwhile wait and then form batches
+ * together.  No counterpart in the reference. */
+int nn_server_hold(nn_server *s, int32_t hold);
 const char *nn_server_last_error(void);
 
 /* ---- learned phase-count / log-K initialiser (no counterpart in the reference) ----

@@ -231,7 +231,7 @@ EXPORTED_SYMBOLS = (
     "nn_mlp_loss_grad", "nn_mlp_train_step", "nn_mlp_predict", "nn_mlp_hints",
     "nn_mlp_get_params", "nn_mlp_set_params", "nn_assembly_info",
     "nn_server_create", "nn_server_destroy", "nn_server_submit_step", "nn_server_wait",
-    "nn_server_release", "nn_server_get_stats", "nn_server_last_error",
+    "nn_server_release", "nn_server_get_stats", "nn_server_last_error", "nn_server_hold",
 )
 
 # Kernel kinds of nn_context_kernel_time.

@@ -414,10 +414,10 @@ __global__ void __launch_bounds__(kBlock) phaseUpdateReflashKernel(
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
     int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
     unsigned int *__restrict__ list, const unsigned int *__restrict__ pendingList,
-    StabRecord<NC> *__restrict__ records)
+    StabRecord<NC> *__restrict__ records, const unsigned int *__restrict__ order)
 {
     const int64_t count = int64_t(q->reflash);
-    const unsigned int *reflashList = pendingList + n;
+    const unsigned int *reflashList = order ? order : pendingList + n;
     drainCounter(q, &q->reflashHead, count, [&](int64_t i) {
         const int64_t s = int64_t(reflashList[i]);
         const int r = phaseUpdateCell<NC, Well, kModeDefer>(*fp, *op, T[s], primary, phaseState, s,
@@ -430,15 +430,11 @@ __global__ void __launch_bounds__(kBlock) phaseUpdateReflashKernel(
 // Phase-update sort key: presence, whether an active saturation sits at or
 // below the removal probe, then (T, P, water fraction of the oil slot).
 template <int NC, bool Well>
-__global__ void phaseUpdateKeyKernel(int64_t n, const double *__restrict__ T,
-                                     const double *__restrict__ primary,
-                                     const double *__restrict__ phaseState,
-                                     unsigned int *__restrict__ keys, unsigned int *__restrict__ idx)
+__device__ inline unsigned int phaseUpdateKey(int64_t s, const double *__restrict__ T,
+                                              const double *__restrict__ primary,
+                                              const double *__restrict__ phaseState)
 {
     using L = Layout<NC, Well>;
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= n)
-        return;
     const double flag = phaseState[s * L::NPS + L::flag];
     const unsigned int presence = static_cast<unsigned int>(flag > 0.0 && flag < 7.5 ? flag + 0.5 : 0.0) & 7u;
     unsigned int removed = 0;
@@ -452,8 +448,45 @@ __global__ void phaseUpdateKeyKernel(int64_t n, const double *__restrict__ T,
     k |= quantize(T[s], 250.0, 750.0, 10) << 18;
     k |= quantize(lp, 11.5, 18.5, 10) << 8;
     k |= quantize(primary[s * L::NP + L::compBegin(0)], 0.0, 1.0, 8);
-    keys[s] = k;
+    return k;
+}
+
+template <int NC, bool Well>
+__global__ void phaseUpdateKeyKernel(int64_t n, const double *__restrict__ T,
+                                     const double *__restrict__ primary,
+                                     const double *__restrict__ phaseState,
+                                     unsigned int *__restrict__ keys, unsigned int *__restrict__ idx)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    keys[s] = phaseUpdateKey<NC, Well>(s, T, primary, phaseState);
     idx[s] = static_cast<unsigned int>(s);
+}
+
+// Keys of the re-flash pass's cells (the held count is read on the device):
+// position i < count holds reflashList[i] under its phase-update key (at most
+// 0xFFFFFFFE), every later position sorts last.
+template <int NC, bool Well>
+__global__ void reflashKeyKernel(int64_t n, const WorkQueue *__restrict__ q,
+                                 const unsigned int *__restrict__ reflashList,
+                                 const double *__restrict__ T, const double *__restrict__ primary,
+                                 const double *__restrict__ phaseState,
+                                 unsigned int *__restrict__ keys, unsigned int *__restrict__ idx)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    if (i >= int64_t(q->reflash))
+    {
+        keys[i] = 0xFFFFFFFFu;
+        idx[i] = 0u;
+        return;
+    }
+    const unsigned int s = reflashList[i];
+    const unsigned int k = phaseUpdateKey<NC, Well>(int64_t(s), T, primary, phaseState);
+    keys[i] = k < 0xFFFFFFFFu ? k : 0xFFFFFFFEu;
+    idx[i] = s;
 }
 
 template <int NC, bool Well>
@@ -753,7 +786,8 @@ hipError_t Launch<NC>::phaseUpdate(bool well, unsigned grid, unsigned coopGrid, 
                                    int32_t *status, uint8_t *flags, WorkQueue *q,
                                    unsigned int *list, const unsigned int *order,
                                    unsigned int *pendingList, void *records,
-                                   hipEvent_t splitA, hipEvent_t splitB)
+                                   const ReflashOrder *reorder, hipEvent_t splitA,
+                                   hipEvent_t splitB)
 {
     StabRecord<NC> *recs = static_cast<StabRecord<NC> *>(records);
     if (!pendingList)
@@ -782,12 +816,29 @@ hipError_t Launch<NC>::phaseUpdate(bool well, unsigned grid, unsigned coopGrid, 
                 fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
+        const unsigned int *reflashOrder = nullptr;
+        if (reorder)
+        {
+            const unsigned keyGrid = static_cast<unsigned>((n + 255) / 256);
+            if (well)
+                reflashKeyKernel<NC, true><<<keyGrid, 256, 0, st>>>(
+                    n, q, pendingList + n, T, primary, phaseState, reorder->keys, reorder->idx);
+            else
+                reflashKeyKernel<NC, false><<<keyGrid, 256, 0, st>>>(
+                    n, q, pendingList + n, T, primary, phaseState, reorder->keys, reorder->idx);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+            if ((e = reorder->sort(reorder->arg, st, &reflashOrder)) != hipSuccess)
+                return e;
+        }
         if (well)
             phaseUpdateReflashKernel<NC, true><<<grid, kBlock, 0, st>>>(
-                fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs);
+                fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs,
+                reflashOrder);
         else
             phaseUpdateReflashKernel<NC, false><<<grid, kBlock, 0, st>>>(
-                fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs);
+                fp, op, n, T, primary, phaseState, status, flags, q, list, pendingList, recs,
+                reflashOrder);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
     }

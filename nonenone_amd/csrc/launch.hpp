@@ -53,6 +53,18 @@ struct WorkQueue
     unsigned int reflashHead; // next item of the re-flash pass
 };
 
+// Processing order of the re-flash pass: the key kernel writes n keys and cell
+// indices (held cells first, in list order; every other position keyed last)
+// into keys/idx, and sort(arg, stream, &order) returns them sorted by key, so
+// neighbouring positions hold cells of similar conditions.
+struct ReflashOrder
+{
+    unsigned int *keys;
+    unsigned int *idx;
+    hipError_t (*sort)(void *arg, hipStream_t st, const unsigned int **order);
+    void *arg;
+};
+
 // Per-state hints of the learned initialiser (both may be null): hint[s] is
 // NN_HINT_*, logK is n x NC.
 struct FlashHint
@@ -89,13 +101,15 @@ struct Launch
     // must re-flash go to a last pass of their own (`pendingList + n`) — all
     // before the cooperative pass.  `pendingList` holds 2n entries, `records`
     // n StabRecord<NC>.
+    // `reorder` (may be null): sorts the re-flash pass's cells by condition first.
     static hipError_t phaseUpdate(bool well, unsigned grid, unsigned coopGrid, hipStream_t st,
                                   const FluidParams *fp, const FlashOptions *op, int64_t n,
                                   const double *T, double *primary, double *phaseState,
                                   int32_t *status, uint8_t *flags, WorkQueue *q,
                                   unsigned int *list, const unsigned int *order,
                                   unsigned int *pendingList, void *records,
-                                  hipEvent_t splitA, hipEvent_t splitB);
+                                  const ReflashOrder *reorder, hipEvent_t splitA,
+                                  hipEvent_t splitB);
     // Sort keys (and identity indices) for the processing order of a batch.
     static hipError_t flashKeys(hipStream_t st, int64_t n, const double *P, const double *T,
                                 const double *z, const uint8_t *hint, unsigned int *keys,

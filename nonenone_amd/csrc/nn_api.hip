@@ -280,6 +280,18 @@ hipError_t sortOrder(nn_context *ctx, int64_t n, hipStream_t st, const unsigned 
     return e;
 }
 
+struct ReflashSortArg
+{
+    nn_context *ctx;
+    int64_t n;
+};
+
+hipError_t reflashSort(void *arg, hipStream_t st, const unsigned int **order)
+{
+    const auto *a = static_cast<const ReflashSortArg *>(arg);
+    return sortOrder(a->ctx, a->n, st, order);
+}
+
 unsigned int coopGrid(const nn_context *ctx)
 {
     return static_cast<unsigned int>(ctx->coopBlocks > 0
@@ -630,6 +642,11 @@ int nn_context_set_schedule(nn_context *ctx, const nn_schedule *s)
     if (s->persistent_blocks_per_cu < 1 || s->persistent_blocks_per_cu > 64 ||
         s->coop_blocks_per_cu < 1 || s->coop_blocks_per_cu > 64)
         return fail(NN_E_INVALID_ARGUMENT, "blocks per CU must be in 1..64");
+    // The plain form runs every recovery path in one lane; its six-component
+    // build returned results that differ from the two-pass form's (which
+    // matches the reference), so it is offered for three components only.
+    if (s->plain && ctx->host.nc != 3)
+        return fail(NN_E_UNSUPPORTED, "the plain one-kernel form is built for three components");
     ctx->sched = *s;
     ctx->sched.ordered = s->ordered ? 1 : 0;
     ctx->sched.wave_claim = s->wave_claim ? 1 : 0;
@@ -783,13 +800,18 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
         if (ctx->lockstep)
             if (const int rc = ensureRecords(ctx, n, sizeof(StabRecord<NC>)); rc != NN_OK)
                 return rc;
+        // an ordered batch also sorts its re-flash pass (the sort buffers are free
+        // again once the first pass has run)
+        ReflashSortArg sortArg{ctx, n};
+        const ReflashOrder reorder{ctx->dKeys[0], ctx->dIdx[0], reflashSort, &sortArg};
         LaunchTimer t(ctx, NN_KERNEL_PHASE_UPDATE, st);
         t.makeSplit();
         NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, persistentGrid(ctx, n),
                                         coopGrid(ctx), st, ctx->dFluid, ctx->dOpts, n, d_T,
                                         d_primary, d_phase_state, d_status, d_flags, ctx->dQueue,
                                         ctx->dList, order, ctx->lockstep ? ctx->dPending : nullptr,
-                                        ctx->lockstep ? ctx->dRecords : nullptr, t.splitA,
+                                        ctx->lockstep ? ctx->dRecords : nullptr,
+                                        ordered && ctx->lockstep ? &reorder : nullptr, t.splitA,
                                         t.splitB)));
         t.doneSplit(NN_KERNEL_PHASE_UPDATE_COOP);
         return NN_OK;

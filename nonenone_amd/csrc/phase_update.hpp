@@ -466,14 +466,16 @@ struct PhaseUpdater
                     return flags | kFlagStableReduced;
                 }
             }
-            if constexpr (Mode == kModeDefer)
+            if constexpr (Mode == kModeDefer && NC != 3)
             {
                 // The lane-per-cell device build returned wrong statuses, flags and
                 // states for cells on this path (restricted flash, stability test,
                 // then the unrestricted re-flash in one lane) at six components,
                 // while the host build of the same code and the wavefront-
                 // cooperative device build were right: such cells go to the
-                // cooperative pass, which redoes the update from the inputs.
+                // cooperative pass, which redoes the update from the inputs.  At
+                // three components the lane build of the path matches the
+                // reference on every parity grid (PR, SW, PR-CPA), and stays.
                 err.raise(kErrDefer);
                 return flags;
             }

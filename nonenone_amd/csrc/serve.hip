@@ -161,6 +161,7 @@ struct nn_server
     std::condition_variable cvWork, cvDone;
     std::thread worker;
     bool stopping = false;
+    bool held = false;  // nn_server_hold: no new batches while set
     int device = 0;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
@@ -422,13 +423,14 @@ struct nn_server
             }
             if (any)
                 cvDone.notify_all();
-            while (int(running.size()) < kMaxBatchesInFlight && !pending.empty() && admit())
+            while (!held && int(running.size()) < kMaxBatchesInFlight && !pending.empty() &&
+                   admit())
             {
             }
             if (stopping && pending.empty() && running.empty())
                 break;
             if (running.empty())
-                cvWork.wait(lk, [&] { return stopping || !pending.empty(); });
+                cvWork.wait(lk, [&] { return stopping || (!held && !pending.empty()); });
             else
                 cvWork.wait_for(lk, std::chrono::microseconds(20));
         }
@@ -552,6 +554,18 @@ int nn_server_get_stats(nn_server *s, nn_server_stats *out)
     return NN_OK;
 }
 
+int nn_server_hold(nn_server *s, int32_t hold)
+{
+    if (!s)
+        return serveFail(NN_E_INVALID_ARGUMENT, "null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->held = hold != 0;
+    }
+    s->cvWork.notify_all();
+    return NN_OK;
+}
+
 int nn_server_destroy(nn_server *s)
 {
     if (!s)
@@ -559,6 +573,7 @@ int nn_server_destroy(nn_server *s)
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stopping = true;
+        s->held = false;  // pending steps still run before the worker ends
     }
     s->cvWork.notify_all();
     if (s->worker.joinable())

@@ -55,9 +55,10 @@ def declare(L) -> None:
     L.nn_server_wait.argtypes = [_P, _I64, ctypes.POINTER(StepResult)]
     L.nn_server_release.argtypes = [_P, _I64]
     L.nn_server_get_stats.argtypes = [_P, ctypes.POINTER(ServerStats)]
+    L.nn_server_hold.argtypes = [_P, _I32]
     L.nn_server_last_error.restype = ctypes.c_char_p
     for name in ("nn_server_create", "nn_server_destroy", "nn_server_submit_step",
-                 "nn_server_wait", "nn_server_release", "nn_server_get_stats"):
+                 "nn_server_wait", "nn_server_release", "nn_server_get_stats", "nn_server_hold"):
         getattr(L, name).restype = ctypes.c_int
 
 
@@ -137,6 +138,11 @@ class Server:
     def release(self, ticket: int) -> None:
         self._check(self.lib.nn_server_release(self.handle, int(ticket)), "nn_server_release")
 
+    def hold(self, on: bool) -> None:
+        """While held, the server issues no new batch; steps submitted meanwhile
+        form batches together once released."""
+        self._check(self.lib.nn_server_hold(self.handle, int(bool(on))), "nn_server_hold")
+
     def stats(self) -> dict:
         s = ServerStats()
         self._check(self.lib.nn_server_get_stats(self.handle, ctypes.byref(s)),
@@ -206,19 +212,30 @@ class TraceResult:
     nonok: int
 
 
-def run_trace(server: Server, members: list, arrivals, want_bsr: bool = True) -> TraceResult:
+def run_trace(server: Server, members: list, arrivals, want_bsr: bool = True,
+              hold_start: bool = False) -> TraceResult:
     """Submit one step of members[m] at each (t, m) of `arrivals` (seconds from the
-    start, increasing), wait for all, and collect latencies and checksums."""
+    start, increasing), wait for all, and collect latencies and checksums.
+    `hold_start`: the steps arriving at t = 0 are all queued before the server
+    issues anything (so they batch together however the threads are scheduled)."""
     for m in members:
         m.reset()
     torch.cuda.synchronize()
     tickets = []
+    if hold_start:
+        server.hold(True)
+    held = hold_start
     t0 = time.perf_counter()
     for t, mi in arrivals:
+        if held and t > 0.0:
+            server.hold(False)
+            held = False
         delay = t - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
         tickets.append((mi, server.submit(mi, members[mi], want_bsr=want_bsr)))
+    if held:
+        server.hold(False)
     lat, sums, cells, nonok = [], [], 0, 0
     step_of = {}
     end, first = 0.0, float("inf")

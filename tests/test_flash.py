@@ -371,3 +371,26 @@ def test_host_build_of_cpa_matches_general_fixture(tmp_path):
     assert np.array_equal(rows[:, 0].astype(np.int32), g["status"])
     dz = np.abs(rows[:, 1] - g["Z"]) / np.abs(g["Z"])
     assert dz.max() <= 1e-12, dz.max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["flash_nc3_pr", "flash_nc3_sw", "flash_nc3_cpa"])
+def test_gpu_plain_form_is_bit_identical_to_scheduled(name):
+    """The plain one-kernel baseline form (one work-item per state, index order)
+    gives the scheduled two-pass form's results bit for bit; it is built for three
+    components and refused for six."""
+    import torch
+
+    fluid = CASES[name]()
+    P, T, z = batched_flash_states(4096, fluid, seed=13, near_critical_fraction=0.1)
+    fl = _flash_gpu(fluid)
+    sched = flashcmp.to_numpy(fl.flash(P, T, z))
+    fl.set_schedule(plain=1)
+    plain = flashcmp.to_numpy(fl.flash(P, T, z))
+    torch.cuda.synchronize()
+    for key in ("presence", "status", "beta", "x", "Z"):
+        assert np.array_equal(np.asarray(plain[key]).view(np.uint8),
+                              np.asarray(sched[key]).view(np.uint8)), key
+    six = _flash_gpu(fluids.six_component("PR"))
+    with pytest.raises(RuntimeError, match="three components"):
+        six.set_schedule(plain=1)

@@ -23,8 +23,8 @@ def test_gpu_results_are_the_same_alone_and_in_the_mix():
     # nc3-PR members' phase updates merge; then a Poisson trace of mixed arrivals
     arrivals = [(0.0, m) for _ in range(3) for m in range(len(members))]
     arrivals += [(t + 0.05, m) for t, m in poisson_arrivals(12, len(members), 2000.0, seed=3)]
-    alone = run_trace(Server(1, 8 << 30), members, arrivals)
-    mixed = run_trace(Server(16, 8 << 30), members, arrivals)
+    alone = run_trace(Server(1, 8 << 30), members, arrivals, hold_start=True)
+    mixed = run_trace(Server(16, 8 << 30), members, arrivals, hold_start=True)
     assert alone.stats["peak_batch_steps"] == 1 and alone.stats["peak_merged_cells"] == 0
     assert mixed.stats["peak_batch_steps"] > 1, mixed.stats
     assert mixed.stats["peak_merged_cells"] > 0, mixed.stats   # two nc3-PR members merged
