@@ -527,7 +527,13 @@ def serving_workload(args) -> dict:
         f"p50 {np.percentile(plain.latencies_ms, 50):.1f} ms, "
         f"p99 {np.percentile(plain.latencies_ms, 99):.1f} ms")
     cont = run_trace(Server(args.batch_steps, budget), members, arrivals)
-    same = dict(plain.checksums) == dict(cont.checksums)
+    pc, cc = dict(plain.checksums), dict(cont.checksums)
+    same = pc == cc
+    differing = sorted({members[k[0]].kind for k in pc if pc[k] != cc.get(k)})
+    n_diff = sum(1 for k in pc if pc[k] != cc.get(k))
+    if not same:
+        log(f"steps whose results differ between the two forms: {n_diff} of {len(pc)} "
+            f"(member kinds {differing})")
     log(f"continuous batching (<= {args.batch_steps} steps per batch): {cont.cells} cell-steps in "
         f"{cont.makespan_s:.3f}s, p50 {np.percentile(cont.latencies_ms, 50):.1f} ms, "
         f"p99 {np.percentile(cont.latencies_ms, 99):.1f} ms; stats {cont.stats}; "
@@ -546,6 +552,8 @@ def serving_workload(args) -> dict:
                                f"continuous batching (<= {args.batch_steps} steps per batch, "
                                f"phase updates merged per fluid), {args.budget_gb} GB budget",
                    "members": len(members), "member_cells": total_cells},
+        "results_identical_to_plain_form": same,
+        "steps_differing_from_plain_form": n_diff, "kinds_differing": differing,
         "p50_ms": float(np.percentile(cont.latencies_ms, 50)),
         "p99_ms": float(np.percentile(cont.latencies_ms, 99)),
         "plain_form": {"value": plain_rate, "p50_ms": float(np.percentile(plain.latencies_ms, 50)),
