@@ -135,13 +135,16 @@ def build_workload(nx, ny, nz, seed=0, eos="CPA"):
 def run_step(fl, asm, s):
     s["prim"].copy_(s["prim_pert"])
     s["ps"].copy_(s["ps_pert"])
-    st, _ = fl.update_phase_state(s["T"], s["prim"], s["ps"], has_well_unknown=True)
+    st, flags = fl.update_phase_state(s["T"], s["prim"], s["ps"], has_well_unknown=True)
     asm.evaluate(DT, s["prim"], s["ps"], s["acc"], out=s["out"])
+    s["status"], s["flags"] = st, flags
     return st
 
 
-def cpu_baseline(fluid, host, dims, sub=(50, 50, 50), target_s=12.0):
-    """The reference's phase update + assembly (oracle/_ref build) on a sub-block."""
+def cpu_baseline(fluid, host, dims, sub=(50, 50, 50), target_s=12.0, gpu=None):
+    """The reference's phase update + assembly (oracle/_ref build) on a sub-block.
+    `gpu` (the timed step's phase-update outputs as host arrays): also compared
+    with the reference's on the sub-block's cells (parity at the bench's scale)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import refharness
 
@@ -177,9 +180,21 @@ def cpu_baseline(fluid, host, dims, sub=(50, 50, 50), target_s=12.0):
         t_total += time.perf_counter() - t0
         passes += 1
     n = len(cells)
-    return {"value": n * passes / t_total, "unit": "cells/s", "cores": cores, "kind": "reference",
-            "sample": f"{sx}x{sy}x{sz} sub-block ({n} cells) of the same workload, {passes} passes "
-                      f"of phase update + residual + BSR Jacobian in {t_total:.1f}s"}
+    out = {"value": n * passes / t_total, "unit": "cells/s", "cores": cores, "kind": "reference",
+           "sample": f"{sx}x{sy}x{sz} sub-block ({n} cells) of the same workload, {passes} passes "
+                     f"of phase update + residual + BSR Jacobian in {t_total:.1f}s"}
+    if gpu is not None:
+        r_prim, r_ps, r_st, r_fl = ref.phase_update(T, prim0, ps0, has_well=True, threads=cores)
+        g_prim, g_ps = gpu["prim"][cells], gpu["ps"][cells]
+        d = np.abs(g_prim - r_prim) / np.maximum(np.maximum(np.abs(g_prim), np.abs(r_prim)), 1.0)
+        per_cell = d.max(axis=1)
+        out["parity"] = {
+            "cells": int(n), "status_mismatches": int((gpu["status"][cells] != r_st).sum()),
+            "flag_mismatches": int((gpu["flags"][cells] != r_fl).sum()),
+            "label_mismatches": int((g_ps[:, 0] != r_ps[:, 0]).sum()),
+            "primary_max_rel_diff": float(per_cell.max()),
+            "cells_above_1e-10": int((per_cell > 1e-10).sum())}
+    return out
 
 
 FLASH_CASES = (("nc3", "PR"), ("nc3", "SW"), ("nc6", "PR"), ("nc6", "SW"))
@@ -649,7 +664,9 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()
-        cpu = cpu_baseline(fluid, host, (args.nx, args.ny, args.nz))
+        gpu = {"prim": s["prim"].cpu().numpy(), "ps": s["ps"].cpu().numpy(),
+               "status": s["status"].cpu().numpy(), "flags": s["flags"].cpu().numpy()}
+        cpu = cpu_baseline(fluid, host, (args.nx, args.ny, args.nz), gpu=gpu)
         log(f"cpu baseline {time.time() - t0:.1f}s: {cpu}")
 
     ms_per_step = elapsed * 1e3 / args.steps
