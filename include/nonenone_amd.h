@@ -293,7 +293,8 @@ typedef struct nn_assembly nn_assembly;
 
 /* Uploads topology (device pointers, kept by reference, must outlive the
  * handle) and precomputes, per connection, the BSR slot of its reciprocal block
- * and, per cell, the slot of its diagonal block.  Synchronous. */
+ * and, per cell, the slot of its diagonal block.  Synchronous.  Built for fluids
+ * of at most 6 components (NN_E_UNSUPPORTED above). */
 int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
                        const int64_t *d_conn_offsets, const int64_t *d_conn_neighbor,
                        const double *d_conn_trans, const double *d_conn_gdz,
