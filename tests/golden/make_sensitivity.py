@@ -79,25 +79,29 @@ def parse(stdout: str) -> dict:
 
 
 def update_sets(exe: Path, tmp: Path) -> dict:
-    """The phase-update inputs of the SW grid test (60 x 20, seed 21, no well)."""
+    """The phase-update inputs of the SW grid tests (60 x 20, seed 21, no well):
+    three and six components."""
     import assemblycmp
     import refharness
     from nonenone_amd._abi import FlashOptions
 
-    fluid = fluids.h2o_co2_nc10("SW")
-    ref = refharness.Reference(fluid, FlashOptions.defaults())
-    w, _, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 60, 20, 21, ref)
-    (tmp / "desc.bin").write_bytes(bytes(fluid))
-    with open(tmp / "cells.bin", "wb") as f:
-        np.array([len(cur), 0], np.int64).tofile(f)
-        np.ascontiguousarray(w["T"], np.float64).tofile(f)
-        np.ascontiguousarray(cur, np.float64).tofile(f)
-        np.ascontiguousarray(hist_ps, np.float64).tofile(f)
-    res = subprocess.run([str(exe), "update", str(tmp / "desc.bin"), str(tmp / "cells.bin")],
-                         check=True, capture_output=True, text=True)
-    flagged = parse(res.stdout)
-    print(f"update_sw_60x20_seed21: {len(flagged)} of {len(cur)} cells libm-conditioned", flush=True)
-    return {"update_sw_60x20_seed21": {"n": int(len(cur)), "conditioned": flagged}}
+    out = {}
+    for name, fluid in (("update_sw_60x20_seed21", fluids.h2o_co2_nc10("SW")),
+                        ("update_sw6_60x20_seed21", fluids.six_component("SW"))):
+        ref = refharness.Reference(fluid, FlashOptions.defaults())
+        w, _, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 60, 20, 21, ref)
+        (tmp / "desc.bin").write_bytes(bytes(fluid))
+        with open(tmp / "cells.bin", "wb") as f:
+            np.array([len(cur), 0], np.int64).tofile(f)
+            np.ascontiguousarray(w["T"], np.float64).tofile(f)
+            np.ascontiguousarray(cur, np.float64).tofile(f)
+            np.ascontiguousarray(hist_ps, np.float64).tofile(f)
+        res = subprocess.run([str(exe), "update", str(tmp / "desc.bin"), str(tmp / "cells.bin")],
+                             check=True, capture_output=True, text=True)
+        flagged = parse(res.stdout)
+        print(f"{name}: {len(flagged)} of {len(cur)} cells libm-conditioned", flush=True)
+        out[name] = {"n": int(len(cur)), "conditioned": flagged}
+    return out
 
 
 def main() -> None:

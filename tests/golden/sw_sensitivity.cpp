@@ -202,6 +202,7 @@ int main(int argc, char **argv)
         int rc = 6;
         if (d.nc == 3 && hdr[1]) rc = runUpdate<3, true>(fp0, op0, hdr[0], fc);
         else if (d.nc == 3) rc = runUpdate<3, false>(fp0, op0, hdr[0], fc);
+        else if (d.nc == 6 && !hdr[1]) rc = runUpdate<6, false>(fp0, op0, hdr[0], fc);
         std::fclose(fc);
         return rc;
     }

@@ -93,12 +93,9 @@ def test_gpu_assembly_matches_fixture():
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
-@pytest.mark.parametrize("model,has_well", [
-    ("PR", False), ("SW", False), ("PR", True), ("CPA", False), ("CPA", True), ("PR6", False),
-    pytest.param("SW6", False, marks=pytest.mark.xfail(
-        strict=False, reason="six-component Soreide-Whitson: 6 of 1,200 cells' updated primary "
-        "values differ from the reference build (statuses, flags and labels match); not shown "
-        "to be conditioned by the math library, cause open"))])
+@pytest.mark.parametrize("model,has_well", [("PR", False), ("SW", False), ("PR", True),
+                                            ("CPA", False), ("CPA", True), ("PR6", False),
+                                            ("SW6", False)])
 def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     import torch
     from nonenone_amd.assembly import NaturalAssembly
@@ -128,7 +125,8 @@ def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     d = np.abs(m_prim - r_prim) / np.maximum(np.maximum(np.abs(m_prim), np.abs(r_prim)), 1.0)
     unexplained, conditioned = libm_conditioning.split_value_failures(
         d.max(axis=1), np.ones(len(d), bool), 1e-10,
-        "update_sw_60x20_seed21" if model == "SW" and not has_well else "")
+        {"SW": "update_sw_60x20_seed21", "SW6": "update_sw6_60x20_seed21"}.get(model, "")
+        if not has_well else "")
     print(f"phase-update primary max rel diff {d.max():.3e}; above tolerance: {len(conditioned)} "
           f"libm-conditioned {conditioned}, {len(unexplained)} other {unexplained}")
     assert not unexplained
