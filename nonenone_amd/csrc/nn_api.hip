@@ -228,6 +228,8 @@ int ensureList(nn_context *ctx, int64_t n)
 }
 
 constexpr int64_t kOrderMinItems = 4096;  // smaller batches run in index order
+// hipcub's radix sort takes an int item count: larger batches run in index order.
+constexpr int64_t kOrderMaxItems = int64_t(0x7FFFFFFF);
 
 void releaseSort(nn_context *ctx)
 {
@@ -246,6 +248,8 @@ void releaseSort(nn_context *ctx)
 // Key/index buffers and radix-sort scratch for n items (synchronous when it grows).
 int ensureSort(nn_context *ctx, int64_t n)
 {
+    if (n > kOrderMaxItems)
+        return fail(NN_E_INVALID_ARGUMENT, "ordered batches hold at most 2^31-1 items");
     if (n <= ctx->sortCap)
         return NN_OK;
     releaseSort(ctx);
@@ -584,7 +588,7 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
     }
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
-    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems;
+    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems && n <= kOrderMaxItems;
     if (ordered)
         if (const int rc = ensureSort(ctx, n); rc != NN_OK)
             return rc;
@@ -780,7 +784,7 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
-    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems;
+    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems && n <= kOrderMaxItems;
     if (ordered)
         if (const int rc = ensureSort(ctx, n); rc != NN_OK)
             return rc;

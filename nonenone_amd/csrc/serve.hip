@@ -31,9 +31,14 @@
 #include <vector>
 
 #include "../../include/nonenone_amd.h"
+#include "serve_plan.hpp"
 
 namespace
 {
+
+using nn::align256;
+using nn::GroupPlan;
+using nn::stagingBytes;
 
 constexpr int kMaxBatchesInFlight = 2;
 
@@ -43,17 +48,6 @@ int serveFail(int code, const char *msg)
 {
     std::snprintf(g_serveError, sizeof(g_serveError), "%s", msg);
     return code;
-}
-
-size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
-
-// Staging of a merged phase update: T, primary, phase state, status, flags.
-size_t stagingBytes(int64_t cells, int np, int nps)
-{
-    return align256(size_t(cells) * sizeof(double)) +
-           align256(size_t(cells) * np * sizeof(double)) +
-           align256(size_t(cells) * nps * sizeof(double)) +
-           align256(size_t(cells) * sizeof(int32_t)) + align256(size_t(cells));
 }
 
 struct Buffers
@@ -98,43 +92,6 @@ struct Batch
     std::vector<Buffers> staging;
     size_t stagingBytes = 0;
     hipEvent_t done = nullptr;
-};
-
-// Merge groups of a batch in first-appearance order; fluid_group < 0 stands alone.
-struct GroupPlan
-{
-    std::vector<std::vector<int64_t>> groups;
-    std::unordered_map<int64_t, size_t> index;
-    std::vector<int64_t> cells;
-    std::vector<int> np, nps;
-
-    void add(int64_t ticket, int64_t key, int64_t ncell, int np_, int nps_)
-    {
-        if (key >= 0)
-        {
-            auto it = index.find(key);
-            if (it != index.end())
-            {
-                groups[it->second].push_back(ticket);
-                cells[it->second] += ncell;
-                return;
-            }
-            index[key] = groups.size();
-        }
-        groups.push_back({ticket});
-        cells.push_back(ncell);
-        np.push_back(np_);
-        nps.push_back(nps_);
-    }
-
-    size_t staging() const
-    {
-        size_t b = 0;
-        for (size_t g = 0; g < groups.size(); ++g)
-            if (groups[g].size() > 1)
-                b += stagingBytes(cells[g], np[g], nps[g]);
-        return b;
-    }
 };
 
 } // namespace
@@ -343,7 +300,7 @@ struct nn_server
                 continue;
             }
             GroupPlan trial = plan;
-            trial.add(*it, r.req.fluid_group, r.ncell, r.np, r.nps);
+            trial.add(*it, r.req.fluid_group, r.ncell, r.np, r.nps, r.req.has_well_unknown);
             const size_t need = outputs + r.bytes + trial.staging();
             if (inUse + need > budget && (inflight > 0 || !b.tickets.empty()))
                 break;  // later requests wait: no starvation of large steps

@@ -46,14 +46,15 @@ def structural_zero_mismatch(mine, ref, rel=1e-12):
     return int(np.sum(differ & ~cancel))
 
 
-def reference_grid_states(fluid, nx, ny, seed, ref, has_well=False):
-    """Seeded nx x ny x 1 grid (dx = dy = 10 m, dz = 5 m), history state from the
-    reference's flash, then the seeded Newton-like perturbation.  Host arrays:
-    (workload dict, history primary, history phase state, perturbed primary)."""
+def reference_grid_states(fluid, nx, ny, seed, ref, has_well=False, nz=1, hydrostatic=False):
+    """Seeded nx x ny x nz grid (dx = dy = 10 m, dz = 5 m; with `hydrostatic`, P gains
+    rho_w g depth down the k index), history state from the reference's flash, then
+    the seeded Newton-like perturbation.  Host arrays: (workload dict, history
+    primary, history phase state, perturbed primary)."""
     from nonenone_amd.assembly import assign_flash_result, perturb_newton_state
     from nonenone_amd.workloads import grid_workload
 
-    w = grid_workload(nx, ny, 1, 10.0, 10.0, 5.0, fluid, seed=seed)
+    w = grid_workload(nx, ny, nz, 10.0, 10.0, 5.0, fluid, seed=seed, hydrostatic=hydrostatic)
     fl = ref.flash(w["P"], w["T"], w["z"], threads=8)
     keep = fl["converged"] == 1
     assert keep.mean() > 0.95
