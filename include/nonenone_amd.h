@@ -271,6 +271,35 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown,
                           double *d_phase_state, int32_t *d_status,
                           uint8_t *d_flags, void *stream);
 
+/*
+ * All-or-none phase update: NaturalPetscRuntime::updateStateWithReport
+ * (models/include/natural/petsc/detail/natural_petsc_runtime_state.inc:166-334).
+ * Every cell is updated on copies in the caller's workspace (d_work_primary
+ * n x np, d_work_phase_state n x nps); only when no cell fails are the copies
+ * written back over d_primary / d_phase_state, otherwise both stay untouched
+ * (the reference's transaction: a recoverable thermodynamic failure -- and here
+ * any non-OK cell status -- leaves no half-updated state).  The report, written
+ * to device memory (d_report), follows StateUpdateReport: the first failing cell
+ * in index order with its status and flag bits, the cells updated before it
+ * (all n on success) and how many of those were StableReducedSet updates
+ * (flag bit 5).  d_status / d_flags receive every cell's outcome.  Asynchronous
+ * on `stream`, no host synchronisation and no memset (capturable in a graph).
+ */
+typedef struct nn_update_report {
+    int64_t first_failure_cell;    /* -1: every cell updated and committed       */
+    int64_t updated_cells;         /* cells before the first failure (n on success) */
+    int64_t stable_reduced_cells;  /* of those, StableReducedSet updates          */
+    int32_t succeeded;             /* 1: committed                                */
+    int32_t first_failure_status;  /* NN_STATUS_* of the first failing cell       */
+    int32_t first_failure_flags;   /* its flag bits (as d_flags)                  */
+    int32_t reserved;
+} nn_update_report;
+int nn_phase_update_transaction(nn_context *ctx, int64_t n, int32_t has_well_unknown,
+                                const double *d_T, double *d_primary, double *d_phase_state,
+                                double *d_work_primary, double *d_work_phase_state,
+                                int32_t *d_status, uint8_t *d_flags,
+                                nn_update_report *d_report, void *stream);
+
 /* ---- Natural assembly -------------------------------------------------- */
 
 /* Host-side setup (no device work): 7-point structured TPFA connections in the
@@ -288,6 +317,22 @@ int nn_structured_topology(int32_t nx, int32_t ny, int32_t nz,
                            double *conn_trans, double *conn_gdz,
                            int64_t *row_offsets, int64_t *col_blocks,
                            int64_t *nconn, int64_t *nnzb);
+
+/* Host routine (no device work): the BSR values of nn_assembly_evaluate (copied
+ * to the host) regrouped by column, in the order of the reference's one
+ * MatSetValuesBlocked(J, 1 + n_conn, rows, 1, &cell, values, ADD_VALUES) call per
+ * cell (natural_petsc_runtime_jacobian.inc:279-299): for cell c the rows are
+ * [c, its face neighbours in connection order] (rows[column_offsets[c] ..
+ * column_offsets[c+1])) and the values the blocks (row, c), each np x np
+ * row-major, in the same order.  values: nnzb x np x np, rows: nnzb,
+ * column_offsets: ncell + 1.  (A caller that creates the BAIJ matrix from the
+ * cached pattern can hand the unregrouped values to
+ * MatSeqBAIJSetPreallocationCSR with MAT_ROW_ORIENTED: they are already
+ * nnzb x np x np row-major blocks in rowOffsets/columnBlocks order.) */
+int nn_bsr_cell_columns(int64_t ncell, int32_t np, const int64_t *conn_offsets,
+                        const int64_t *conn_neighbor, const int64_t *row_offsets,
+                        const int64_t *col_blocks, const double *bsr, double *values,
+                        int64_t *rows, int64_t *column_offsets);
 
 typedef struct nn_assembly nn_assembly;
 
@@ -314,14 +359,47 @@ int nn_assembly_accumulation(nn_assembly *a, const double *d_primary,
  * R2 + R3: residual F (ncell x np) and BSR Jacobian values (nnzb x np x np,
  * row-major blocks, in the cached pattern's rowOffsets/columnBlocks order),
  * exactly the values the reference writes with MatSetValuesBlocked
- * (scaling disabled).  d_residual or d_bsr may be NULL to skip that part.
- * d_decisions (ncell... nconn x 3 uint8 upwind flags, may be NULL).
+ * (with the handle's scaling, nn_assembly_set_scaling; none by default).
+ * d_residual or d_bsr may be NULL to skip that part.  d_decisions: nconn x 3
+ * uint8 upwind flags (0 = interior upwind), may be NULL.
  */
 int nn_assembly_evaluate(nn_assembly *a, double dt, double fugacity_scaling,
                          const double *d_primary, const double *d_phase_state,
                          const double *d_accum_prev, double *d_residual,
                          double *d_bsr, uint8_t *d_decisions,
                          int32_t *d_status, void *stream);
+
+/*
+ * Equation and variable scaling of the Natural system (NaturalScalingOptions,
+ * models/include/natural/petsc/natural_scaling.hpp:20-47; factors
+ * models/include/natural/petsc/detail/natural_petsc_runtime_scaling.inc:8-89,
+ * applied as natural_petsc_runtime_jacobian.inc:196-213, 262-268 and
+ * natural_petsc_runtime_common.inc:345-356).  With `enabled`, every later
+ * nn_assembly_evaluate on the handle writes
+ *   residual[c][r]  = R_r * rowScale(r, c)
+ *   J[r][q]         = (dR_r/dx_q * rowScale(r, c)) * varScale(q)
+ * where rowScale is 1/mass_residual_scale on conservation rows,
+ * 1/fugacity_residual_scale on both fugacity blocks, 1/closure_residual_scale on
+ * the closure row and, on the well-control row, d_well_row_scale[c] when given
+ * (the reference's representative-well rule: 1/max(|target|,
+ * rate_well_residual_floor) for a cell of an active rate-controlled well,
+ * 1/pressure_scale otherwise), else 1/pressure_scale; varScale is
+ * pressure_scale on p and the well pressure, saturation_scale on saturations,
+ * composition_scale on mole fractions.  d_well_row_scale (ncell doubles, device)
+ * is kept by reference.  Disabled (the default) = every factor 1.
+ */
+typedef struct nn_scaling {
+    int32_t enabled;                  /* 0 */
+    double pressure_scale;            /* 1e7 Pa */
+    double composition_scale;         /* 1 */
+    double saturation_scale;          /* 1 */
+    double mass_residual_scale;       /* 1 */
+    double fugacity_residual_scale;   /* 1 */
+    double closure_residual_scale;    /* 1 */
+    double rate_well_residual_floor;  /* 1 (used by the caller's well-row rule) */
+} nn_scaling;
+void nn_scaling_default(nn_scaling *s);
+int nn_assembly_set_scaling(nn_assembly *a, const nn_scaling *s, const double *d_well_row_scale);
 
 /* Sizes of an assembly handle: its context, cells, BSR blocks and primary
  * variables per cell (no counterpart in the reference). */

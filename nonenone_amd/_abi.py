@@ -13,6 +13,11 @@ from pathlib import Path
 NN_MAX_NC = 8
 
 NN_OK = 0
+NN_E_INVALID_ARGUMENT = -1
+NN_E_UNSUPPORTED = -2
+NN_E_HIP = -3
+NN_E_NO_DEVICE = -4
+NN_E_OUT_OF_MEMORY = -5
 NN_STATUS_OK = 0
 NN_STATUS_NOT_CONVERGED = 1
 NN_STATUS_THERMO_ERROR = 2
@@ -115,6 +120,41 @@ class FlashOut(ctypes.Structure):
     ]
 
 
+class UpdateReport(ctypes.Structure):
+    """nn_update_report: StateUpdateReport of the all-or-none phase update
+    (natural_petsc_runtime_state.inc:166-334)."""
+
+    _fields_ = [
+        ("first_failure_cell", _I64),
+        ("updated_cells", _I64),
+        ("stable_reduced_cells", _I64),
+        ("succeeded", _I32),
+        ("first_failure_status", _I32),
+        ("first_failure_flags", _I32),
+        ("reserved", _I32),
+    ]
+
+
+class Scaling(ctypes.Structure):
+    """nn_scaling: the Natural runtime's equation/variable scaling
+    (NaturalScalingOptions, models/include/natural/petsc/natural_scaling.hpp:20-47)."""
+
+    _fields_ = [
+        ("enabled", _I32),
+        ("pressure_scale", _D),
+        ("composition_scale", _D),
+        ("saturation_scale", _D),
+        ("mass_residual_scale", _D),
+        ("fugacity_residual_scale", _D),
+        ("closure_residual_scale", _D),
+        ("rate_well_residual_floor", _D),
+    ]
+
+    @classmethod
+    def defaults(cls) -> "Scaling":
+        return cls(0, 1.0e7, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
+
+
 class Schedule(ctypes.Structure):
     """nn_schedule: launch shapes of the flash-like batches of a context."""
 
@@ -140,6 +180,11 @@ _PRODUCT_LIB = None
 
 
 def library_path() -> Path:
+    """The in-tree build; NN_LIBRARY names another build of the same sources (a
+    variant compiled with different build parameters, for timing)."""
+    override = os.environ.get("NN_LIBRARY")
+    if override:
+        return Path(override).resolve()
     return Path(__file__).resolve().parent / "libnonenone_amd.so"
 
 
@@ -203,6 +248,12 @@ def _declare(L: ctypes.CDLL) -> None:
                                     _P]
     L.nn_mlp_predict.argtypes = [_P, _I64, _P, _P, _P, _P]
     L.nn_mlp_hints.argtypes = [_P, _I64, _P, _P, ctypes.c_float, _I32, _P, _P, _P]
+    L.nn_phase_update_transaction.argtypes = [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P,
+                                              _P]
+    L.nn_bsr_cell_columns.argtypes = [_I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.nn_scaling_default.argtypes = [ctypes.POINTER(Scaling)]
+    L.nn_scaling_default.restype = None
+    L.nn_assembly_set_scaling.argtypes = [_P, ctypes.POINTER(Scaling), _P]
     L.nn_assembly_info.argtypes = [_P, ctypes.POINTER(_P), ctypes.POINTER(_I64),
                                    ctypes.POINTER(_I64), ctypes.POINTER(_I32)]
     L.nn_assembly_info.restype = ctypes.c_int
@@ -232,6 +283,8 @@ EXPORTED_SYMBOLS = (
     "nn_mlp_get_params", "nn_mlp_set_params", "nn_assembly_info",
     "nn_server_create", "nn_server_destroy", "nn_server_submit_step", "nn_server_wait",
     "nn_server_release", "nn_server_get_stats", "nn_server_last_error", "nn_server_hold",
+    "nn_scaling_default", "nn_assembly_set_scaling",
+    "nn_phase_update_transaction", "nn_bsr_cell_columns",
 )
 
 # Kernel kinds of nn_context_kernel_time.

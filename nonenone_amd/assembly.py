@@ -63,6 +63,22 @@ class NaturalAssembly:
         except Exception:
             pass
 
+    def set_scaling(self, enabled=True, well_row_scale=None, **factors) -> None:
+        """The runtime's equation/variable scaling for later evaluations
+        (natural_petsc_runtime_scaling.inc:8-89): factors named as nn_scaling's
+        fields (reference defaults otherwise); well_row_scale: per-cell factor of
+        the well-control row (host or device array), None = 1/pressure_scale."""
+        sc = _abi.Scaling.defaults()
+        sc.enabled = 1 if enabled else 0
+        for name, value in factors.items():
+            setattr(sc, name, float(value))
+        self._well_row_scale = (None if well_row_scale is None else
+                                torch.as_tensor(np.ascontiguousarray(well_row_scale),
+                                                dtype=torch.float64).to(self.d["T"].device))
+        _abi.check(self.lib.nn_assembly_set_scaling(self.handle, ctypes.byref(sc),
+                                                    _ptr(self._well_row_scale)),
+                   "nn_assembly_set_scaling")
+
     def accumulation(self, primary: torch.Tensor, phase_state: torch.Tensor) -> torch.Tensor:
         acc = torch.empty(self.ncell, self.nc, dtype=torch.float64, device=primary.device)
         _abi.check(self.lib.nn_assembly_accumulation(self.handle, _ptr(primary), _ptr(phase_state),

@@ -25,8 +25,10 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
-OUT = HERE / "libnonenone_amd.so"
-BUILD_DIR = HERE.parent / "_build"
+OUT = Path(os.environ.get("NN_LIB_OUT", str(HERE / "libnonenone_amd.so")))
+BUILD_DIR = Path(os.environ.get("NN_BUILD_DIR", str(HERE.parent / "_build")))
+# Minimum wavefronts per SIMD of the per-state kernels (csrc/launch.hpp).
+WAVES_PER_EU = os.environ.get("NN_WAVES_PER_EU")
 ARCH = os.environ.get("NN_OFFLOAD_ARCH", "gfx950")
 UNITS = ("nn_api.hip", "nn_nc2.hip", "nn_nc3.hip", "nn_nc4.hip", "nn_nc5.hip", "nn_nc6.hip", "nn_nc8.hip",
          "mlp.hip", "serve.hip")
@@ -50,7 +52,8 @@ def _filtered(stderr: str) -> str:
 
 def _compile(unit: str) -> tuple[str, Path, str]:
     obj = BUILD_DIR / (unit + ".o")
-    cmd = [hipcc(), *FLAGS, "-Rpass-analysis=kernel-resource-usage",
+    extra = [f"-DNN_WAVES_PER_EU={int(WAVES_PER_EU)}"] if WAVES_PER_EU else []
+    cmd = [hipcc(), *FLAGS, *extra, "-Rpass-analysis=kernel-resource-usage",
            "-I", str(HERE.parent / "include"), "-c", str(CSRC / unit), "-o", str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
@@ -68,11 +71,26 @@ def _compile(unit: str) -> tuple[str, Path, str]:
     return unit, obj, warnings
 
 
-def build(verbose: bool = True) -> Path:
+def _up_to_date(unit: str) -> bool:
+    """True when the unit's object is newer than the unit and every header it may
+    include (csrc/*.hpp, include/*.h*)."""
+    obj = BUILD_DIR / (unit + ".o")
+    if not obj.exists():
+        return False
+    deps = [CSRC / unit, *CSRC.glob("*.hpp"), *(HERE.parent / "include").glob("*.h*")]
+    return all(d.stat().st_mtime < obj.stat().st_mtime for d in deps)
+
+
+def build(verbose: bool = True, incremental: bool = False) -> Path:
+    """Compile every unit and link.  incremental=True (or NN_BUILD_INCREMENTAL=1)
+    reuses objects that are newer than their sources and every header."""
     BUILD_DIR.mkdir(exist_ok=True)
     t0 = time.time()
-    with ThreadPoolExecutor(max_workers=len(UNITS)) as pool:
-        results = list(pool.map(_compile, UNITS))
+    incremental = incremental or os.environ.get("NN_BUILD_INCREMENTAL") == "1"
+    todo = [u for u in UNITS if not (incremental and _up_to_date(u))]
+    with ThreadPoolExecutor(max_workers=max(1, len(todo))) as pool:
+        done = {u: r for u, r in zip(todo, pool.map(_compile, todo))}
+    results = [done.get(u, (u, BUILD_DIR / (u + ".o"), "")) for u in UNITS]
     tmp = OUT.with_suffix(".so.tmp")
     link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC",
             *(str(o) for _, o, _ in results), "-o", str(tmp)]
@@ -89,4 +107,4 @@ def build(verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)

@@ -14,7 +14,7 @@ namespace nn
 
 // ------------------------------------------------------------ E1-E5 kernel
 template <int NC>
-__global__ void __launch_bounds__(kBlock) eosPhaseKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) eosPhaseKernel(
     const FluidParams *__restrict__ fp, int64_t n, const double *__restrict__ P,
     const double *__restrict__ T, const double *__restrict__ x, const int32_t *__restrict__ role,
     double *__restrict__ Z, double *__restrict__ lnphi, double *__restrict__ fug,
@@ -209,7 +209,7 @@ __device__ inline void drainQueue(WorkQueue *q, int64_t n, F &&body)
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kBlock) flashKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) flashKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
     const uint8_t *__restrict__ allowed, const double *__restrict__ seed, FlashHint hint,
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) flashKernel(
 
 // The baseline form: one work-item per state in index order, no queue, no deferral.
 template <int NC>
-__global__ void __launch_bounds__(kBlock) flashPlainKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) flashPlainKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
     const uint8_t *__restrict__ allowed, const double *__restrict__ seed, FlashHint hint,
@@ -273,7 +273,7 @@ __global__ void flashKeyKernel(int64_t n, const double *__restrict__ P,
 
 // One wavefront per deferred state (blocks of 64 = one wave).
 template <int NC>
-__global__ void __launch_bounds__(kBlock) flashCoopKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) flashCoopKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
     const uint8_t *__restrict__ allowed, const double *__restrict__ seed, FlashHint hint,
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(kBlock) flashCoopKernel(
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kBlock) stabilityKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) stabilityKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
     const uint8_t *__restrict__ active, const double *__restrict__ xin, uint8_t *stable,
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(kBlock) stabilityKernel(
 
 // ---------------------------------------------------- phase-update kernel
 template <int NC, bool Well>
-__global__ void __launch_bounds__(kBlock) phaseUpdateKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) phaseUpdateKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
     int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(kBlock) phaseUpdateKernel(
 // step, the residual evaluation, all lanes of the wavefront take together; a
 // lane whose cell is complete claims the next recorded cell.
 template <int NC>
-__global__ void __launch_bounds__(kBlock) stabilityTrialKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) stabilityTrialKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
     WorkQueue *__restrict__ q, const unsigned int *__restrict__ pendingList,
     StabRecord<NC> *__restrict__ records)
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(kBlock) stabilityTrialKernel(
 // Finishing pass of the recorded cells: the phase update again, with the
 // stability test replayed from the lockstep trials' results.
 template <int NC, bool Well>
-__global__ void __launch_bounds__(kBlock) phaseUpdateReplayKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) phaseUpdateReplayKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
     int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(kBlock) phaseUpdateReplayKernel(
 // Re-flash pass: the replayed cells whose missing phase turned out unstable,
 // together, so that the wavefronts of this costly unrestricted flash are full.
 template <int NC, bool Well>
-__global__ void __launch_bounds__(kBlock) phaseUpdateReflashKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) phaseUpdateReflashKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
     int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
@@ -490,7 +490,7 @@ __global__ void reflashKeyKernel(int64_t n, const WorkQueue *__restrict__ q,
 }
 
 template <int NC, bool Well>
-__global__ void __launch_bounds__(kBlock) phaseUpdateCoopKernel(
+__global__ void __launch_bounds__(kBlock, kWavesPerEu) phaseUpdateCoopKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op,
     const double *__restrict__ T, double *__restrict__ primary, double *__restrict__ phaseState,
     int32_t *__restrict__ status, uint8_t *__restrict__ flags, WorkQueue *__restrict__ q,
@@ -524,7 +524,7 @@ __device__ bool loadCell(const FluidParams &f, const AssemblyView &v, const doub
 
 // Value-only property pass: the record a neighbour's face flux reads.
 template <int NC, bool Well>
-__global__ void __launch_bounds__(256) cellRecordKernel(
+__global__ void __launch_bounds__(256, kWavesPerEu) cellRecordKernel(
     const FluidParams *__restrict__ fp, AssemblyView v, const double *__restrict__ primary,
     const double *__restrict__ phaseState, double *__restrict__ accum, int32_t *__restrict__ status)
 {
@@ -570,7 +570,7 @@ __global__ void __launch_bounds__(256) cellRecordKernel(
 
 // Residual + Jacobian column of one cell (residual.inc:59-120, jacobian.inc:20-237).
 template <int NC, bool Well>
-__global__ void __launch_bounds__(64) assembleKernel(
+__global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
     const FluidParams *__restrict__ fp, AssemblyView v, double dt, double fugScale,
     const double *__restrict__ primary, const double *__restrict__ phaseState,
     const double *__restrict__ accumPrev, double *__restrict__ residual, double *__restrict__ bsr,
@@ -615,10 +615,17 @@ __global__ void __launch_bounds__(64) assembleKernel(
     S res[NP];
     localResidual<NC, Well, S>(st, pr, accumPrev + c * NC, v.volume[c], dt, fugScale, res);
 
+    // row scale of each equation of this cell (natural_petsc_runtime_scaling.inc:26-89)
+    double rowScale[NP];
+    for (int r = 0; r < NP; ++r) rowScale[r] = v.rowScale[r];
+    if constexpr (Well)
+        if (v.wellRowScale)
+            rowScale[NP - 1] = v.wellRowScale[c];
+    // jacobian.inc:262-268: (derivative * rowScale) * variableScale
     double *self = bsr ? bsr + v.selfSlot[c] * NP * NP : nullptr;
     if (self)
         for (int r = 0; r < NP; ++r)
-            for (int q = 0; q < NP; ++q) self[r * NP + q] = res[r].d[q];
+            for (int q = 0; q < NP; ++q) self[r * NP + q] = res[r].d[q] * rowScale[r] * v.varScale[q];
 
     double outward[NC];
     for (int k = 0; k < NC; ++k) outward[k] = 0.0;
@@ -683,22 +690,24 @@ __global__ void __launch_bounds__(64) assembleKernel(
                     flux[i] += total * rec[(10 + q * NC + i) * n + nb];
             }
         }
+        // jacobian.inc:196-213: mass rows carry one scale in every cell
         double *rb = bsr + v.recipSlot[k] * NP * NP;
         for (int i = 0; i < NC; ++i)
             for (int q = 0; q < NP; ++q)
             {
-                self[i * NP + q] += flux[i].d[q];
-                rb[i * NP + q] = -flux[i].d[q];
+                self[i * NP + q] += flux[i].d[q] * rowScale[i] * v.varScale[q];
+                rb[i * NP + q] = -flux[i].d[q] * v.rowScale[i] * v.varScale[q];
             }
         for (int r = NC; r < NP; ++r)
             for (int q = 0; q < NP; ++q) rb[r * NP + q] = 0.0;
     }
     if (residual)
     {
+        // natural_petsc_runtime_common.inc:345-356
         for (int i = 0; i < NC; ++i)
-            residual[c * NP + i] = res[i].v + outward[i];
+            residual[c * NP + i] = (res[i].v + outward[i]) * rowScale[i];
         for (int r = NC; r < NP; ++r)
-            residual[c * NP + r] = res[r].v;
+            residual[c * NP + r] = res[r].v * rowScale[r];
     }
     if (status)
         status[c] = NN_STATUS_OK;

@@ -16,12 +16,30 @@ namespace nn
 
 constexpr int kBlock = 64;  // one wavefront per block: divergent per-state control flow
 
+// Minimum wavefronts per SIMD that every per-state kernel is compiled for.  The
+// flash, stability and phase-update kernels share out-of-line solver functions
+// (Flash::flashTwo, flashThree, thermoForRole, ...); a callee's register budget
+// follows the launch bounds only when every kernel that calls it asks for the
+// same occupancy, so all kernels of this file carry the same value.  At 1 the
+// shared callees take the full 512 registers (one wavefront per SIMD).  At 2 the
+// PR-CPA phase update of a 500,000-cell grid ran 7 % faster, but scratch grows
+// with occupancy (private segment x resident waves per queue), and a second
+// hardware queue (another stream, e.g. a graph-capture stream) could then not
+// be given its scratch: HSA_STATUS_ERROR_OUT_OF_RESOURCES.  Build parameter
+// (nonenone_amd/build.py, NN_WAVES_PER_EU); 1 by default.
+#ifndef NN_WAVES_PER_EU
+#define NN_WAVES_PER_EU 1
+#endif
+constexpr int kWavesPerEu = NN_WAVES_PER_EU;
+
 // Largest component count the Natural assembly is built for.  Its forward-mode
 // AD carries (3 NC + 1)-wide dual numbers per lane; at eight components the
 // assembly kernel's private segment is about 85 KB per lane, and its dispatch
 // stopped the queue with HSA_STATUS_ERROR_OUT_OF_RESOURCES (scratch), so larger
 // fluids get flash and phase updates but no assembly.
 constexpr int kAssemblyMaxNc = 6;
+
+constexpr int kMaxNp = 3 * NN_MAX_NC + 2;  // primary variables with a well unknown
 
 struct AssemblyView
 {
@@ -36,6 +54,12 @@ struct AssemblyView
     const int64_t *selfSlot;   // BSR slot of block (c, c)
     const int64_t *recipSlot;  // per connection: BSR slot of block (neighbour, c)
     double *record;            // neighbour property records, field-major (SoA)
+    // Equation/variable scaling of the Natural runtime (all 1 when disabled):
+    // residual[r] *= rowScale[r]; J[r][q] = (dR_r/dx_q * rowScale[r]) * varScale[q].
+    // wellRowScale (per cell, may be null) replaces rowScale of the well-control row.
+    const double *wellRowScale;
+    double rowScale[kMaxNp];
+    double varScale[kMaxNp];
 };
 
 // Counters of a two-pass (lane-per-item, then wavefront-per-deferred-item) launch.

@@ -393,6 +393,74 @@ FlashOptions toOptions(const nn_flash_options &o)
     return r;
 }
 
+// ---- all-or-none phase update (natural_petsc_runtime_state.inc:166-334) ----
+// The transaction's device record: the lowest failing cell index (atomicMin) and
+// the report the caller reads.
+__global__ void txnInitKernel(nn_update_report *r)
+{
+    r->first_failure_cell = -1;
+    r->updated_cells = 0;
+    r->stable_reduced_cells = 0;
+    r->succeeded = 0;
+    r->first_failure_status = NN_STATUS_OK;
+    r->first_failure_flags = 0;
+    r->reserved = 0;
+}
+
+// first failing cell = lowest index whose status is not OK; the 64-bit slot
+// first_failure_cell holds n (no failure) before the reduction.
+__global__ void txnFirstFailureKernel(int64_t n, const int32_t *__restrict__ status,
+                                      unsigned long long *__restrict__ first)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c < n && status[c] != NN_STATUS_OK)
+        atomicMin(first, static_cast<unsigned long long>(c));
+}
+
+// StableReducedSet updates before the first failure (the reference counts the
+// cells it updated before it stopped).
+__global__ void txnCountKernel(int64_t n, const uint8_t *__restrict__ flags,
+                               const unsigned long long *__restrict__ first,
+                               unsigned long long *__restrict__ stable)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t limit = static_cast<int64_t>(*first);
+    if (c < n && c < limit && (flags[c] & 0x20))
+        atomicAdd(stable, 1ull);
+}
+
+__global__ void txnReportKernel(int64_t n, const int32_t *__restrict__ status,
+                                const uint8_t *__restrict__ flags, nn_update_report *r)
+{
+    const int64_t first = r->first_failure_cell;   // n when nothing failed
+    r->updated_cells = first;
+    if (first >= n)
+    {
+        r->first_failure_cell = -1;
+        r->succeeded = 1;
+        return;
+    }
+    r->succeeded = 0;
+    r->first_failure_status = status[first];
+    r->first_failure_flags = flags[first];
+}
+
+// Commit: the updated copies replace the caller's vectors only on success.
+__global__ void txnCommitKernel(int64_t count, const nn_update_report *__restrict__ r,
+                                const double *__restrict__ src, double *__restrict__ dst)
+{
+    if (!r->succeeded)
+        return;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += int64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void txnSetKernel(unsigned long long *slot, unsigned long long value)
+{
+    *slot = value;
+}
+
 } // namespace
 
 // ===================================================================== C ABI
@@ -822,6 +890,63 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
     });
 }
 
+int nn_phase_update_transaction(nn_context *ctx, int64_t n, int32_t has_well_unknown,
+                                const double *d_T, double *d_primary, double *d_phase_state,
+                                double *d_work_primary, double *d_work_phase_state,
+                                int32_t *d_status, uint8_t *d_flags, nn_update_report *d_report,
+                                void *stream)
+{
+    if (!ctx || n < 0)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    if (!d_T || !d_primary || !d_phase_state || !d_work_primary || !d_work_phase_state ||
+        !d_status || !d_flags || !d_report)
+        return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
+    if (d_work_primary == d_primary || d_work_phase_state == d_phase_state)
+        return fail(NN_E_INVALID_ARGUMENT, "the workspace must not alias the committed vectors");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nc = ctx->host.nc;
+    const int64_t np = 3 * nc + 1 + (has_well_unknown ? 1 : 0);
+    const int64_t nps = 3 * nc + 8;
+    txnInitKernel<<<1, 1, 0, st>>>(d_report);
+    NN_HIP(hipGetLastError());
+    if (n == 0)
+    {
+        txnReportKernel<<<1, 1, 0, st>>>(0, d_status, d_flags, d_report);
+        NN_HIP(hipGetLastError());
+        return NN_OK;
+    }
+    NN_HIP(hipMemcpyAsync(d_work_primary, d_primary, sizeof(double) * size_t(n * np),
+                          hipMemcpyDeviceToDevice, st));
+    NN_HIP(hipMemcpyAsync(d_work_phase_state, d_phase_state, sizeof(double) * size_t(n * nps),
+                          hipMemcpyDeviceToDevice, st));
+    if (const int rc = nn_phase_update_batch(ctx, n, has_well_unknown, d_T, d_work_primary,
+                                             d_work_phase_state, d_status, d_flags, stream);
+        rc != NN_OK)
+        return rc;
+    // report fields as 64-bit scratch of the reduction: first failing index, count
+    auto *first = reinterpret_cast<unsigned long long *>(&d_report->first_failure_cell);
+    auto *stable = reinterpret_cast<unsigned long long *>(&d_report->stable_reduced_cells);
+    txnSetKernel<<<1, 1, 0, st>>>(first, static_cast<unsigned long long>(n));
+    NN_HIP(hipGetLastError());
+    const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+    txnFirstFailureKernel<<<grid, 256, 0, st>>>(n, d_status, first);
+    NN_HIP(hipGetLastError());
+    txnCountKernel<<<grid, 256, 0, st>>>(n, d_flags, first, stable);
+    NN_HIP(hipGetLastError());
+    txnReportKernel<<<1, 1, 0, st>>>(n, d_status, d_flags, d_report);
+    NN_HIP(hipGetLastError());
+    const auto commitGrid = [](int64_t count) {
+        return static_cast<unsigned>(std::min<int64_t>((count + 255) / 256, 4096));
+    };
+    txnCommitKernel<<<commitGrid(n * np), 256, 0, st>>>(n * np, d_report, d_work_primary,
+                                                        d_primary);
+    NN_HIP(hipGetLastError());
+    txnCommitKernel<<<commitGrid(n * nps), 256, 0, st>>>(n * nps, d_report, d_work_phase_state,
+                                                         d_phase_state);
+    NN_HIP(hipGetLastError());
+    return NN_OK;
+}
+
 // ------------------------------------------------------------------ assembly
 int nn_structured_topology(int32_t nx, int32_t ny, int32_t nz, const double *dx, const double *dy,
                            const double *dz, const double *perm, int64_t *conn_offsets,
@@ -899,6 +1024,44 @@ int nn_structured_topology(int32_t nx, int32_t ny, int32_t nz, const double *dx,
     if (row_offsets) row_offsets[ncell] = nb_;
     *nconn = nc_;
     *nnzb = nb_;
+    return NN_OK;
+}
+
+int nn_bsr_cell_columns(int64_t ncell, int32_t np, const int64_t *conn_offsets,
+                        const int64_t *conn_neighbor, const int64_t *row_offsets,
+                        const int64_t *col_blocks, const double *bsr, double *values,
+                        int64_t *rows, int64_t *column_offsets)
+{
+    if (ncell < 0 || np <= 0 || !conn_offsets || !conn_neighbor || !row_offsets || !col_blocks ||
+        !bsr || !values || !rows || !column_offsets)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    const size_t bs2 = size_t(np) * size_t(np);
+    auto slotOf = [&](int64_t row, int64_t col) -> int64_t {
+        const int64_t *b = col_blocks + row_offsets[row], *e = col_blocks + row_offsets[row + 1];
+        const int64_t *it = std::lower_bound(b, e, col);
+        return (it == e || *it != col) ? -1 : int64_t(it - col_blocks);
+    };
+    int64_t out = 0;
+    for (int64_t c = 0; c < ncell; ++c)
+    {
+        column_offsets[c] = out;
+        // natural_petsc_runtime_jacobian.inc:279-299: rows [self, neighbours in
+        // connection order], one column block (the cell), blocks row-major
+        const int64_t count = 1 + conn_offsets[c + 1] - conn_offsets[c];
+        for (int64_t q = 0; q < count; ++q)
+        {
+            const int64_t row = q == 0 ? c : conn_neighbor[conn_offsets[c] + q - 1];
+            if (row < 0 || row >= ncell)
+                return fail(NN_E_INVALID_ARGUMENT, "connection to a cell outside the grid");
+            const int64_t slot = slotOf(row, c);
+            if (slot < 0)
+                return fail(NN_E_INVALID_ARGUMENT, "pattern lacks a block of the cell's column");
+            rows[out] = row;
+            std::memcpy(values + size_t(out) * bs2, bsr + size_t(slot) * bs2, bs2 * sizeof(double));
+            ++out;
+        }
+    }
+    column_offsets[ncell] = out;
     return NN_OK;
 }
 
@@ -1010,8 +1173,71 @@ int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
     if (nconn > 0)
         NN_HIP(hipMemcpy(a->dRecip, recip.data(), sizeof(int64_t) * nconn, hipMemcpyHostToDevice));
     a->view = AssemblyView{ncell, d_conn_offsets, d_conn_neighbor, d_conn_trans, d_conn_gdz,
-                           d_volume, d_porosity, d_T, a->dSelf, a->dRecip, a->dRecord};
+                           d_volume, d_porosity, d_T, a->dSelf, a->dRecip, a->dRecord,
+                           nullptr, {}, {}};
+    for (int k = 0; k < kMaxNp; ++k)
+        a->view.rowScale[k] = a->view.varScale[k] = 1.0;
     *out = a;
+    return NN_OK;
+}
+
+void nn_scaling_default(nn_scaling *s)
+{
+    if (!s)
+        return;
+    s->enabled = 0;
+    s->pressure_scale = 1.0e7;
+    s->composition_scale = 1.0;
+    s->saturation_scale = 1.0;
+    s->mass_residual_scale = 1.0;
+    s->fugacity_residual_scale = 1.0;
+    s->closure_residual_scale = 1.0;
+    s->rate_well_residual_floor = 1.0;
+}
+
+int nn_assembly_set_scaling(nn_assembly *a, const nn_scaling *s, const double *d_well_row_scale)
+{
+    if (!a || !s)
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    const double f[7] = {s->pressure_scale, s->composition_scale, s->saturation_scale,
+                         s->mass_residual_scale, s->fugacity_residual_scale,
+                         s->closure_residual_scale, s->rate_well_residual_floor};
+    for (double v : f)
+        if (!(v > 0.0) || !std::isfinite(v))
+            return fail(NN_E_INVALID_ARGUMENT, "scaling factors must be finite and positive");
+    const int nc = a->ctx->host.nc;
+    const int np = 3 * nc + 1 + (a->hasWell ? 1 : 0);
+    AssemblyView &v = a->view;
+    for (int k = 0; k < kMaxNp; ++k)
+        v.rowScale[k] = v.varScale[k] = 1.0;
+    v.wellRowScale = nullptr;
+    if (!s->enabled)
+        return NN_OK;
+    // natural_petsc_runtime_scaling.inc:8-89 on this layout: equations
+    // mass[0, nc), fugacity [nc, 2nc), water fugacity [2nc, 3nc), closure 3nc,
+    // well control 3nc+1; variables p, compositions [1, 3nc-2), saturations
+    // [3nc-2, 3nc+1), well pressure 3nc+1.
+    for (int e = 0; e < np; ++e)
+    {
+        if (e < nc)
+            v.rowScale[e] = 1.0 / s->mass_residual_scale;
+        else if (e < 3 * nc)
+            v.rowScale[e] = 1.0 / s->fugacity_residual_scale;
+        else if (e == 3 * nc)
+            v.rowScale[e] = 1.0 / s->closure_residual_scale;
+        else
+            v.rowScale[e] = 1.0 / s->pressure_scale;
+    }
+    for (int q = 0; q < np; ++q)
+    {
+        if (q == 0 || q == 3 * nc + 1)
+            v.varScale[q] = s->pressure_scale;
+        else if (q >= 3 * nc - 2 && q <= 3 * nc)
+            v.varScale[q] = s->saturation_scale;
+        else
+            v.varScale[q] = s->composition_scale;
+    }
+    v.wellRowScale = a->hasWell ? d_well_row_scale : nullptr;
     return NN_OK;
 }
 

@@ -231,3 +231,23 @@ class ThreePhaseFlash(Context):
                                                   _ptr(status), _ptr(flags), _stream()),
                    "nn_phase_update_batch")
         return status, flags
+
+    def update_state_with_report(self, T, primary, phase_state, has_well_unknown=False):
+        """NaturalPetscRuntime::updateStateWithReport (natural_petsc_runtime_state.inc:166-334):
+        every cell updated on copies, committed over primary / phase_state only when
+        no cell fails.  Returns (status, flags, report dict)."""
+        T = _dev(T, torch.float64)
+        n = T.shape[0]
+        assert primary.is_contiguous() and phase_state.is_contiguous()
+        status = torch.empty(n, dtype=torch.int32, device=T.device)
+        flags = torch.empty(n, dtype=torch.uint8, device=T.device)
+        work_p = torch.empty_like(primary)
+        work_s = torch.empty_like(phase_state)
+        report = torch.empty(ctypes.sizeof(_abi.UpdateReport), dtype=torch.uint8, device=T.device)
+        _abi.check(self.lib.nn_phase_update_transaction(
+            self.handle, n, int(bool(has_well_unknown)), _ptr(T), _ptr(primary),
+            _ptr(phase_state), _ptr(work_p), _ptr(work_s), _ptr(status), _ptr(flags),
+            _ptr(report), _stream()), "nn_phase_update_transaction")
+        r = _abi.UpdateReport.from_buffer_copy(report.cpu().numpy().tobytes())
+        return status, flags, {name: int(getattr(r, name)) for name, _ in r._fields_
+                               if name != "reserved"}

@@ -321,6 +321,53 @@ int64_t findSlot(const Topology &t, int64_t row, int64_t col)
     return it - t.colBlocks;
 }
 
+// Equation and variable scale factors of the Natural runtime
+// (natural_petsc_runtime_scaling.inc:8-89), restated because they are private
+// members of the PETSc-bound runtime.  `params` = {pressureScale,
+// compositionScale, saturationScale, massResidualScale, fugacityResidualScale,
+// closureResidualScale}; nullptr = scaling disabled (every factor 1).  The
+// well-control row's factor depends on the cell's representative well
+// (rate-controlled: 1 / max(|target|, floor), otherwise 1 / pressureScale):
+// `wellRow` gives it per cell (nullptr = 1 / pressureScale everywhere).
+template <class I>
+struct ScalingRule
+{
+    const double *params = nullptr;
+    const double *wellRow = nullptr;
+
+    double variable(int v) const
+    {
+        if (!params)
+            return 1.0;
+        if (v == I::Primary::pressure || (I::hasWellUnknown && v == I::Primary::wellPressure))
+            return params[0];
+        if (v == I::Primary::liquidSaturation || v == I::Primary::vaporSaturation ||
+            (I::hasWater && v == I::Primary::waterSaturation))
+            return params[2];
+        return params[1];
+    }
+
+    double equation(int e, int64_t cell) const
+    {
+        if (!params)
+            return 1.0;
+        if (e >= I::Equation::massConservationBegin && e < I::Equation::afterMassConservation)
+            return 1.0 / params[3];
+        if (e >= I::Equation::fugacityBegin && e < I::Equation::fugacityBegin + I::numComponents)
+            return 1.0 / params[4];
+        if constexpr (I::fullyCompositionalThreePhase)
+            if (e >= I::Equation::waterFugacityBegin &&
+                e < I::Equation::waterFugacityBegin + I::numComponents)
+                return 1.0 / params[4];
+        if (e == I::Equation::volumeClosure)
+            return 1.0 / params[5];
+        if constexpr (I::hasWellUnknown)
+            if (e == I::Equation::wellControl)
+                return wellRow ? wellRow[cell] : 1.0 / params[0];
+        return 1.0;
+    }
+};
+
 template <int NC, bool Well>
 struct AssemblyRunner
 {
@@ -387,8 +434,9 @@ struct AssemblyRunner
 
     // residual.inc:59-120 and jacobian.inc:20-237 for cells [b, e); each cell
     // writes its residual row, its diagonal block and its reciprocal blocks.
-    void evaluate(Worker &w, double dt, double fugScale, const double *accumPrev,
-                  double *residual, double *bsr, uint8_t *decisionsOut, int64_t b, int64_t e)
+    void evaluate(Worker &w, double dt, double fugScale, const ScalingRule<I> &sc,
+                  const double *accumPrev, double *residual, double *bsr,
+                  uint8_t *decisionsOut, int64_t b, int64_t e)
     {
         constexpr int N = NP;
         std::vector<MPMC::FaceFluxLinearizationDecision<I>> decisions;
@@ -426,14 +474,19 @@ struct AssemblyRunner
                 std::array<Scalar, NC> noWell{};
                 MPMC::addCellConservationTransportAndWell<I>(
                     withFlux, outward, noWell, Scalar(0.0));
+                // natural_petsc_runtime_common.inc:345-356
                 for (int r = 0; r < N; ++r)
-                    residual[c * N + r] = MPMC::scalarValue(withFlux.value[r]);
+                    residual[c * N + r] = MPMC::scalarValue(withFlux.value[r]) * sc.equation(r, c);
             }
             if (!bsr)
                 continue;
+            // jacobian.inc:262-268 (self block), :196-213 (face terms)
             for (int r = 0; r < N; ++r)
+            {
+                const double rowScale = sc.equation(r, c);
                 for (int q = 0; q < N; ++q)
-                    selfBlock[r * N + q] = block.value[r].derivative(q);
+                    selfBlock[r * N + q] = block.value[r].derivative(q) * rowScale * sc.variable(q);
+            }
             for (int64_t k = cb; k < ce; ++k)
             {
                 const int64_t nb = t.connNeighbor[k];
@@ -445,11 +498,14 @@ struct AssemblyRunner
                 for (int q = 0; q < NC; ++q)
                 {
                     const int row = I::Equation::massConservation[q];
+                    const double selfRowScale = sc.equation(row, c);
+                    const double recipRowScale = sc.equation(row, nb);
                     for (int col = 0; col < N; ++col)
                     {
                         const double dv = flux.component[q].derivative(col);
-                        selfBlock[row * N + col] += dv;
-                        recip[row * N + col] = -dv;
+                        const double varScale = sc.variable(col);
+                        selfBlock[row * N + col] += dv * selfRowScale * varScale;
+                        recip[row * N + col] = -dv * recipRowScale * varScale;
                     }
                 }
             }
@@ -462,11 +518,14 @@ struct AssemblyRunner
 
 template <int NC, bool Well>
 int assemblyRun(const Handle &h, const Topology &t, double dt, double fugScale,
+                const double *scaling, const double *wellRowScale,
                 const double *primary, const double *phaseState,
                 const double *accumPrev, double *residual, double *bsr,
                 uint8_t *decisions, double *accumOut, int threads)
 {
     using Runner = AssemblyRunner<NC, Well>;
+    using I = typename Runner::I;
+    const ScalingRule<I> sc{scaling, wellRowScale};
     constexpr int N = Runner::NP;
     Runner runner(h, t);
     if (bsr)
@@ -481,7 +540,7 @@ int assemblyRun(const Handle &h, const Topology &t, double dt, double fugScale,
     if (residual || bsr)
         parallelFor(t.ncell, threads, [&](int, int64_t b, int64_t e) {
             typename Runner::Worker w(h.desc);
-            runner.evaluate(w, dt, fugScale, accumPrev, residual, bsr, decisions, b, e);
+            runner.evaluate(w, dt, fugScale, sc, accumPrev, residual, bsr, decisions, b, e);
         });
     return NN_OK;
 }
@@ -566,14 +625,17 @@ int ref_phase_update(void *hv, int64_t n, int hasWell, const double *T,
     });
 }
 
-int ref_assembly(void *hv, int64_t ncell, int hasWell, const int64_t *connOffsets,
-                 const int64_t *connNeighbor, const double *connTrans,
-                 const double *connGdz, const int64_t *rowOffsets,
-                 const int64_t *colBlocks, const double *volume,
-                 const double *porosity, const double *T, double dt,
-                 double fugScale, const double *primary, const double *phaseState,
-                 const double *accumPrev, double *residual, double *bsr,
-                 uint8_t *decisions, double *accumOut, int threads)
+// ref_assembly with the runtime's equation/variable scaling: `scaling` = the six
+// factors of ScalingRule (nullptr = disabled), `wellRowScale` per cell or nullptr.
+int ref_assembly_scaled(void *hv, int64_t ncell, int hasWell, const int64_t *connOffsets,
+                        const int64_t *connNeighbor, const double *connTrans,
+                        const double *connGdz, const int64_t *rowOffsets,
+                        const int64_t *colBlocks, const double *volume,
+                        const double *porosity, const double *T, double dt,
+                        double fugScale, const double *scaling, const double *wellRowScale,
+                        const double *primary, const double *phaseState,
+                        const double *accumPrev, double *residual, double *bsr,
+                        uint8_t *decisions, double *accumOut, int threads)
 {
     const Handle &h = *static_cast<Handle *>(hv);
     Topology t{ncell, connOffsets, connNeighbor, connTrans, connGdz, rowOffsets,
@@ -583,10 +645,12 @@ int ref_assembly(void *hv, int64_t ncell, int hasWell, const int64_t *connOffset
         return dispatchNc(h.desc.nc, [&](auto nc) {
             constexpr int NC = decltype(nc)::value;
             return hasWell
-                ? assemblyRun<NC, true>(h, t, dt, fugScale, primary, phaseState,
-                                        accumPrev, residual, bsr, decisions, accumOut, threads)
-                : assemblyRun<NC, false>(h, t, dt, fugScale, primary, phaseState,
-                                         accumPrev, residual, bsr, decisions, accumOut, threads);
+                ? assemblyRun<NC, true>(h, t, dt, fugScale, scaling, wellRowScale, primary,
+                                        phaseState, accumPrev, residual, bsr, decisions,
+                                        accumOut, threads)
+                : assemblyRun<NC, false>(h, t, dt, fugScale, scaling, wellRowScale, primary,
+                                         phaseState, accumPrev, residual, bsr, decisions,
+                                         accumOut, threads);
         });
     }
     catch (const std::exception &e)
@@ -594,6 +658,21 @@ int ref_assembly(void *hv, int64_t ncell, int hasWell, const int64_t *connOffset
         std::snprintf(g_error, sizeof(g_error), "%s", e.what());
         return NN_E_INVALID_ARGUMENT;
     }
+}
+
+int ref_assembly(void *hv, int64_t ncell, int hasWell, const int64_t *connOffsets,
+                 const int64_t *connNeighbor, const double *connTrans,
+                 const double *connGdz, const int64_t *rowOffsets,
+                 const int64_t *colBlocks, const double *volume,
+                 const double *porosity, const double *T, double dt,
+                 double fugScale, const double *primary, const double *phaseState,
+                 const double *accumPrev, double *residual, double *bsr,
+                 uint8_t *decisions, double *accumOut, int threads)
+{
+    return ref_assembly_scaled(hv, ncell, hasWell, connOffsets, connNeighbor, connTrans, connGdz,
+                               rowOffsets, colBlocks, volume, porosity, T, dt, fugScale, nullptr,
+                               nullptr, primary, phaseState, accumPrev, residual, bsr, decisions,
+                               accumOut, threads);
 }
 
 } // extern "C"

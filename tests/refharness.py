@@ -37,6 +37,9 @@ def lib():
         L.ref_phase_update.argtypes = [_P, _I64, ctypes.c_int] + [_P] * 5 + [ctypes.c_int]
         L.ref_assembly.argtypes = ([_P, _I64, ctypes.c_int] + [_P] * 9 +
                                    [ctypes.c_double, ctypes.c_double] + [_P] * 7 + [ctypes.c_int])
+        L.ref_assembly_scaled.argtypes = ([_P, _I64, ctypes.c_int] + [_P] * 9 +
+                                          [ctypes.c_double, ctypes.c_double] + [_P] * 9 +
+                                          [ctypes.c_int])
         _lib = L
     return _lib
 
@@ -120,8 +123,12 @@ class Reference:
         return primary, phase_state, status, flags
 
     def assembly(self, topo, volume, porosity, T, primary, phase_state, accum_prev, dt,
-                 fug_scale=1.0, has_well=False, want_residual=True, want_bsr=True, threads=1):
-        """Restated runtime loops around the reference's per-cell kernels."""
+                 fug_scale=1.0, has_well=False, want_residual=True, want_bsr=True, threads=1,
+                 scaling=None, well_row_scale=None):
+        """Restated runtime loops around the reference's per-cell kernels.  scaling:
+        None (disabled) or the six factors (pressure, composition, saturation, mass
+        residual, fugacity residual, closure residual); well_row_scale: per-cell
+        factor of the well-control row or None."""
         n = len(T)
         np_ = primary.shape[1]
         arr = {k: np.ascontiguousarray(v) for k, v in topo.items()}
@@ -137,13 +144,17 @@ class Reference:
         nconn = int(arr["conn_offsets"][-1])
         dec = np.zeros((nconn, 3), np.uint8)
         acc = np.zeros((n, self.nc))
-        rc = self.L.ref_assembly(self.h, n, int(has_well), _p(arr["conn_offsets"]),
-                                 _p(arr["conn_neighbor"]), _p(arr["conn_trans"]),
-                                 _p(arr["conn_gdz"]), _p(arr["row_offsets"]),
-                                 _p(arr["col_blocks"]), _p(arr["volume"]), _p(arr["porosity"]),
-                                 _p(T), float(dt), float(fug_scale), _p(primary),
-                                 _p(phase_state), _p(accum_prev), _p(res), _p(bsr), _p(dec),
-                                 _p(acc), int(threads))
+        sc = None if scaling is None else np.ascontiguousarray(scaling, np.float64)
+        assert sc is None or sc.shape == (6,)
+        wr = None if well_row_scale is None else np.ascontiguousarray(well_row_scale, np.float64)
+        rc = self.L.ref_assembly_scaled(self.h, n, int(has_well), _p(arr["conn_offsets"]),
+                                        _p(arr["conn_neighbor"]), _p(arr["conn_trans"]),
+                                        _p(arr["conn_gdz"]), _p(arr["row_offsets"]),
+                                        _p(arr["col_blocks"]), _p(arr["volume"]),
+                                        _p(arr["porosity"]), _p(T), float(dt), float(fug_scale),
+                                        _p(sc), _p(wr), _p(primary), _p(phase_state),
+                                        _p(accum_prev), _p(res), _p(bsr), _p(dec), _p(acc),
+                                        int(threads))
         if rc != 0:
             raise RuntimeError(f"ref_assembly failed: {rc}")
         return res, bsr, dec, acc

@@ -464,3 +464,104 @@ def test_gpu_jacobian_matches_central_differences_3d(model, has_well):
                       np.rint(r_ps[:, 0]).astype(int), fluid.nc)
     print(f"GPU {model} well={has_well}: {out}")
     assert out["worst"] <= 1.0e-3, out
+
+
+# ------------------------------------------------------------------ scaling
+# factors of nn_scaling other than the defaults, so that every class of row and
+# column is scaled differently: pressure, composition, saturation, mass residual,
+# fugacity residual, closure residual
+SCALING = (2.0e6, 0.5, 0.25, 1.0e-3, 10.0, 2.0)
+
+
+def _scale_vectors(nc, has_well, well_row=None, ncell=1):
+    """Closed form of natural_petsc_runtime_scaling.inc:8-89 on the primary layout:
+    (row scale per cell and equation, variable scale per primary variable)."""
+    p, x, s, mass, fug, closure = SCALING
+    npv = 3 * nc + 1 + (1 if has_well else 0)
+    row = np.empty(npv)
+    row[:nc] = 1.0 / mass
+    row[nc:3 * nc] = 1.0 / fug
+    row[3 * nc] = 1.0 / closure
+    var = np.full(npv, x)
+    var[0] = p
+    var[3 * nc - 2:3 * nc + 1] = s
+    rows = np.repeat(row[None, :], ncell, axis=0)
+    if has_well:
+        row_w = np.full(ncell, 1.0 / p) if well_row is None else np.asarray(well_row)
+        rows[:, -1] = row_w
+        var[-1] = p
+    return rows, var
+
+
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("has_well", [False, True])
+def test_reference_scaled_assembly_is_row_and_column_scaled(has_well):
+    """The harness's scaled assembly (its restatement of the runtime's scaling)
+    against the closed form D_r R and D_r J D_u of its unscaled assembly, whose
+    Jacobian the central-difference test pins: residual entries exactly, Jacobian
+    entries to 1e-14 of the block row (the self block sums face terms after
+    scaling, so the rounding differs in the last bits)."""
+    fluid = _fluid("PR")
+    ref = refharness.Reference(fluid, FlashOptions.defaults())
+    nx, ny, nz = 8, 6, 3
+    w, hist_p, hist_ps, cur = assemblycmp.reference_grid_states(
+        fluid, nx, ny, 45, ref, has_well=has_well, nz=nz, hydrostatic=True)
+    r_prim, r_ps, _, _ = ref.phase_update(w["T"], cur, hist_ps, has_well, threads=8)
+    args = (w["topo"], w["volume"], w["porosity"], w["T"])
+    _, _, _, acc = ref.assembly(*args, hist_p, hist_ps, np.zeros((len(w["T"]), 3)), DT,
+                                has_well=has_well, want_residual=False, want_bsr=False, threads=8)
+    well_row = np.linspace(1e-7, 3e-7, len(w["T"])) if has_well else None
+    res, bsr, dec, _ = ref.assembly(*args, r_prim, r_ps, acc, DT, has_well=has_well, threads=8)
+    s_res, s_bsr, s_dec, _ = ref.assembly(*args, r_prim, r_ps, acc, DT, has_well=has_well,
+                                          threads=8, scaling=SCALING, well_row_scale=well_row)
+    rows, var = _scale_vectors(3, has_well, well_row, len(w["T"]))
+    assert np.array_equal(dec, s_dec)
+    assert np.array_equal(s_res, res * rows)
+    topo = w["topo"]
+    row_of_slot = np.repeat(np.arange(len(w["T"])), np.diff(topo["row_offsets"]))
+    expect = bsr * rows[row_of_slot][:, :, None] * var[None, None, :]
+    err = assemblycmp.bsr_error(s_bsr, expect)
+    assert err.max() <= 1e-14, err.max()
+    assert np.array_equal(s_bsr == 0.0, expect == 0.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("model,has_well", [("PR", True), ("CPA", True), ("SW", False)])
+def test_gpu_scaled_assembly_matches_reference_build(model, has_well):
+    """nn_assembly_set_scaling as the reference's own case enables it
+    (h2o_co2_nc10_2d_benchmark.cpp:264), with factors of every class set apart
+    and a per-cell well-row factor: residual and BSR against the reference
+    harness's scaled assembly on the 3-D grid, and scaling off again restores the
+    unscaled values bit for bit."""
+    import torch
+    from nonenone_amd.assembly import NaturalAssembly
+    from nonenone_amd.flash import ThreePhaseFlash
+
+    fluid, ref, w, hist_p, hist_ps, cur, upd, r_acc = _reference_3d_case(model, has_well)
+    r_prim, r_ps = upd[0], upd[1]
+    n = len(w["T"])
+    well_row = np.linspace(1e-7, 3e-7, n) if has_well else None
+    r_res, r_bsr, _, _ = ref.assembly(w["topo"], w["volume"], w["porosity"], w["T"], r_prim, r_ps,
+                                      r_acc, DT, has_well=has_well, threads=8, scaling=SCALING,
+                                      well_row_scale=well_row)
+    ctx = ThreePhaseFlash(fluid)
+    asm = NaturalAssembly(ctx, w["topo"], w["volume"], w["porosity"], w["T"], has_well)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    plain = {k: v.clone() for k, v in asm.evaluate(DT, dev(r_prim), dev(r_ps), dev(r_acc)).items()}
+    names = ("pressure_scale", "composition_scale", "saturation_scale", "mass_residual_scale",
+             "fugacity_residual_scale", "closure_residual_scale")
+    asm.set_scaling(True, well_row_scale=well_row, **dict(zip(names, SCALING)))
+    out = asm.evaluate(DT, dev(r_prim), dev(r_ps), dev(r_acc))
+    assert np.all(out["status"].cpu().numpy() == 0)
+    re = assemblycmp.residual_error(out["residual"].cpu().numpy(), r_res)
+    je = assemblycmp.bsr_error(out["bsr"].cpu().numpy(), r_bsr)
+    print(f"scaled {model} well={has_well}: residual max rel {re.max():.3e}, "
+          f"jacobian max rel {je.max():.3e}")
+    assert re.max() <= assemblycmp.REL_TOL
+    assert je.max() <= assemblycmp.REL_TOL
+    assert assemblycmp.structural_zero_mismatch(out["bsr"].cpu().numpy(), r_bsr) == 0
+    asm.set_scaling(False)
+    again = asm.evaluate(DT, dev(r_prim), dev(r_ps), dev(r_acc))
+    for k in ("residual", "bsr", "decisions", "status"):
+        assert torch.equal(again[k], plain[k]), k
