@@ -202,6 +202,24 @@ int nn_flash_seeded_batch(nn_context *ctx, int64_t n,
                           const nn_flash_out *out, void *stream);
 
 /*
+ * nn_flash_batch without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The persistent
+ * kernels' work queue and deferred-state list live in a file-scope device array
+ * of the library (256 B + 4 B x NN_STATIC_MAX_STATES = 4 MiB per GPU, held until
+ * the process ends; its address is resolved per device by nn_context_create).
+ * Same kernels and results as nn_flash_batch in index order; the schedule's
+ * sorted order is not applied (it needs sort buffers).  n > NN_STATIC_MAX_STATES
+ * returns NN_E_INVALID_ARGUMENT.  Every call on one GPU shares the array: calls
+ * that overlap on one GPU (other streams, other threads) race and must not be
+ * issued.  Calls no hipMalloc/hipFree and no allocating library routine.
+ */
+#define NN_STATIC_MAX_STATES (1 << 20)
+int nn_flash_batch_static(nn_context *ctx, int64_t n,
+                          const double *d_P, const double *d_T, const double *d_z,
+                          const nn_flash_out *out, void *stream);
+
+/*
  * Launch shapes of the flash-like batches of a context (no counterpart in the
  * reference: how the per-state work maps onto the GPU).  Defaults {16, 4, 1, 0, 0}.
  *   persistent_blocks_per_cu  grid of the lane-per-state pass (64-thread blocks

@@ -213,6 +213,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.nn_last_error.restype = ctypes.c_char_p
     L.nn_eos_phase_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.nn_flash_batch.argtypes = [_P, _I64, _P, _P, _P, ctypes.POINTER(FlashOut), _P]
+    L.nn_flash_batch_static.argtypes = [_P, _I64, _P, _P, _P, ctypes.POINTER(FlashOut), _P]
     L.nn_flash_restricted_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P,
                                             ctypes.POINTER(FlashOut), _P]
     L.nn_stability_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
@@ -283,7 +284,7 @@ EXPORTED_SYMBOLS = (
     "nn_mlp_get_params", "nn_mlp_set_params", "nn_assembly_info",
     "nn_server_create", "nn_server_destroy", "nn_server_submit_step", "nn_server_wait",
     "nn_server_release", "nn_server_get_stats", "nn_server_last_error", "nn_server_hold",
-    "nn_scaling_default", "nn_assembly_set_scaling",
+    "nn_flash_batch_static", "nn_scaling_default", "nn_assembly_set_scaling",
     "nn_phase_update_transaction", "nn_bsr_cell_columns",
 )
 

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <vector>
@@ -317,6 +318,51 @@ unsigned int persistentGrid(const nn_context *ctx, int64_t n)
 // (at most one 64-item tile per block) must stay below 2^32.
 constexpr int64_t kMaxQueueItems = int64_t(0xF0000000u);
 
+// Allocation-free flash (nn_flash_batch_static): its work queue and deferred-index
+// list live in this file-scope device array, one copy per GPU, held until the
+// process ends.  Layout: WorkQueue (padded to 256 bytes), then up to
+// kStaticMaxStates deferred indices.  Host code uses only the per-device address
+// from hipGetSymbolAddress, never the array's name.
+constexpr int64_t kStaticMaxStates = NN_STATIC_MAX_STATES;
+static_assert(NN_STATIC_MAX_STATES == 1048576, "error text states the limit");
+constexpr size_t kStaticQueueBytes = 256;
+static_assert(sizeof(WorkQueue) <= kStaticQueueBytes, "work queue exceeds its static slot");
+constexpr size_t kStaticScratchBytes =
+    kStaticQueueBytes + sizeof(unsigned int) * size_t(kStaticMaxStates);
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void *g_scratchPtr[kMaxDevices] = {};
+
+// Resolves this device's copy of `scratch` once (device current); NN_OK or NN_E_HIP.
+int resolveScratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return fail(NN_E_UNSUPPORTED, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lk(g_scratchMutex);
+    if (g_scratchPtr[device])
+        return NN_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+    {
+        std::snprintf(g_error, sizeof(g_error), "hipGetSymbolAddress(scratch): %s",
+                      hipGetErrorString(e));
+        return NN_E_HIP;
+    }
+    g_scratchPtr[device] = ptr;
+    return NN_OK;
+}
+
+void *scratchPtr(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratchMutex);
+    return g_scratchPtr[device];
+}
+
 FluidParams toParams(const nn_fluid_desc &d)
 {
     FluidParams p{};
@@ -527,6 +573,11 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
     if (!ctx)
         return fail(NN_E_OUT_OF_MEMORY, "host allocation failed");
     NN_HIP(hipGetDevice(&ctx->device));
+    if (const int rc = resolveScratch(ctx->device); rc != NN_OK)
+    {
+        delete ctx;
+        return rc;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->numCUs = prop.multiProcessorCount;
@@ -697,6 +748,50 @@ int nn_flash_seeded_batch(nn_context *ctx, int64_t n, const double *d_P, const d
         return fail(NN_E_INVALID_ARGUMENT, "hint array required");
     return launchFlash(ctx, n, d_P, d_T, d_z, nullptr, nullptr, FlashHint{d_hint, d_log_k}, out,
                        stream);
+}
+
+int nn_flash_batch_static(nn_context *ctx, int64_t n, const double *d_P, const double *d_T,
+                          const double *d_z, const nn_flash_out *out, void *stream)
+{
+    if (!ctx || n < 0 || !out)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    if (n > kStaticMaxStates)
+        return fail(NN_E_INVALID_ARGUMENT,
+                    "nn_flash_batch_static: batch exceeds the static scratch "
+                    "(NN_STATIC_MAX_STATES = 1048576 states)");
+    if (n == 0)
+        return NN_OK;
+    if (!d_P || !d_T || !d_z || !out->presence || !out->converged || !out->status ||
+        !out->iterations || !out->beta || !out->saturation || !out->x || !out->Z ||
+        !out->rho_m || !out->k_go || !out->k_wo)
+        return fail(NN_E_INVALID_ARGUMENT, "null device pointer");
+    char *base = static_cast<char *>(scratchPtr(ctx->device));
+    if (!base)
+        return fail(NN_E_HIP, "static scratch not resolved for this device");
+    WorkQueue *q = reinterpret_cast<WorkQueue *>(base);
+    unsigned int *list = reinterpret_cast<unsigned int *>(base + kStaticQueueBytes);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const nn_flash_out o = *out;
+    const FlashHint none{nullptr, nullptr};
+    return dispatchNc(ctx->host.nc, [&](auto ncTag) {
+        constexpr int NC = decltype(ncTag)::value;
+        if (ctx->sched.plain)
+        {
+            const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+            NN_HIP((Launch<NC>::flash(grid, 0, st, ctx->dFluid, ctx->dOpts, n, d_P, d_T, d_z,
+                                      nullptr, nullptr, none, o, nullptr, nullptr, nullptr, true,
+                                      nullptr, nullptr)));
+            return NN_OK;
+        }
+        // the queue's counters start at zero (a kernel, not a memset: capturable)
+        resetQueueKernel<<<1, 1, 0, st>>>(q, ctx->coopAll ? 1u : 0u,
+                                          ctx->sched.wave_claim ? 1u : 0u);
+        NN_HIP(hipGetLastError());
+        NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), coopGrid(ctx), st, ctx->dFluid,
+                                  ctx->dOpts, n, d_P, d_T, d_z, nullptr, nullptr, none, o, q,
+                                  list, nullptr, false, nullptr, nullptr)));
+        return NN_OK;
+    });
 }
 
 int nn_context_get_schedule(nn_context *ctx, nn_schedule *s)
