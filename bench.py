@@ -77,6 +77,30 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations of one cell's phase update on the PR-CPA headline workload
+# (add, mul, transcendental = 1, fma = 2), measured by the FP64 instruction
+# counters of `bench.py --workload phase_cell_ops` (256 seeded cells, one cell
+# per launch sequence, so one active lane per wavefront): 3.1103e9 over 256 cells.
+# profiles/phase_update_cpa_fp64_ops_single_cell_sample.md
+PHASE_UPDATE_FP64_OPS_PER_CELL = 1.2150e7
+# HBM bytes (L2 <-> memory, FETCH_SIZE x 2 + WRITE_SIZE x 1 against a 2 GiB copy)
+# of one headline step's kernels, from the two PMC passes of
+# profiles/phase_update_natural_assembly_200x200x50_cpa_hbm_traffic.md
+PHASE_UPDATE_TRAFFIC_BYTES = 2.6606e12
+ASSEMBLE_TRAFFIC_BYTES = 2.051e11
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def assemble_bytes(ncell: int, nconn: int, nnzb: int, nc: int, np_: int, nps: int) -> int:
     """Compulsory HBM bytes of one assembly launch: read primary, phase state,
     T, porosity, volume and M^n per cell and (T_ij, g dz) per connection; write
@@ -180,9 +204,24 @@ def cpu_baseline(fluid, host, dims, sub=(50, 50, 50), target_s=12.0, gpu=None):
         t_total += time.perf_counter() - t0
         passes += 1
     n = len(cells)
+    # one core: the first 4,000 cells of the sub-block (a 40 x 2 x 50 corner keeps
+    # its own connections), one pass
+    single = cells.reshape(sz, sy, sx)[:, :2, :40].reshape(-1)
+    topo1 = workloads.structured_topology(min(40, sx), min(2, sy), sz, 10.0, 10.0, 5.0,
+                                          w["perm"][single])
+    t0 = time.perf_counter()
+    p1, s1, _, _ = ref.phase_update(w["T"][single], host["pert"][single], host["ps"][single],
+                                    has_well=True, threads=1)
+    ref.assembly(topo1, w["volume"][single], w["porosity"][single], w["T"][single], p1, s1,
+                 host["acc"][single], DT, has_well=True, threads=1)
+    t1 = time.perf_counter() - t0
     out = {"value": n * passes / t_total, "unit": "cells/s", "cores": cores, "kind": "reference",
+           "cpu_model": cpu_model(),
+           "single_core_value": len(single) / t1,
            "sample": f"{sx}x{sy}x{sz} sub-block ({n} cells) of the same workload, {passes} passes "
-                     f"of phase update + residual + BSR Jacobian in {t_total:.1f}s"}
+                     f"of phase update + residual + BSR Jacobian in {t_total:.1f}s on {cores} "
+                     f"threads (one per core of the box's 16-core share); single core: "
+                     f"{len(single)} cells of it in {t1:.1f}s"}
     if gpu is not None:
         r_prim, r_ps, r_st, r_fl = ref.phase_update(T, prim0, ps0, has_well=True, threads=cores)
         g_prim, g_ps = gpu["prim"][cells], gpu["ps"][cells]
@@ -581,6 +620,99 @@ def serving_workload(args) -> dict:
     }
 
 
+def phase_cell_ops_workload(args) -> dict:
+    """Single-cell phase updates of a seeded sample of the assembly workload's
+    cells, one cell per launch sequence: with one active lane per wavefront the
+    wave-level FP64 instruction counters of a rocprofv3 --pmc run of this mode are
+    the sampled cells' own FP64 operation counts (no divergence, no idle lanes).
+    Same distribution as the headline step (history flash + seeded Newton
+    perturbation on an nx x ny x nz grid of the same fields)."""
+    import torch
+
+    fluid, fl, asm, s, host = build_workload(args.nx, args.ny, args.nz, eos=args.eos)
+    rng = np.random.Generator(np.random.PCG64(12345))
+    cells = np.sort(rng.choice(asm.ncell, args.samples, replace=False))
+    T, prim, ps = s["T"], s["prim_pert"], s["ps_pert"]
+    counts = np.zeros(5, np.int64)
+    t0 = time.time()
+    for k, c in enumerate(cells):
+        p1 = prim[c:c + 1].clone()
+        s1 = ps[c:c + 1].clone()
+        st, flags = fl.update_phase_state(T[c:c + 1], p1, s1, has_well_unknown=True)
+        counts[int(st[0].item())] += 1
+        if (k + 1) % 64 == 0:
+            log(f"single-cell phase updates {k + 1}/{len(cells)} ({time.time() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    return {"metric": "single-cell phase updates (FP64 operation count sample)",
+            "value": len(cells) / (time.time() - t0), "unit": "cells/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic", "vs_baseline": None,
+            "config": {"workload": f"{len(cells)} seeded cells of the {args.nx}x{args.ny}x"
+                                   f"{args.nz} {EOS_NAMES[args.eos]} assembly workload, one "
+                                   "cell per nn_phase_update_batch call", "cells": len(cells)},
+            "statuses": counts.tolist(), "roofline": None, "cpu_baseline": None}
+
+
+def assembly_only_workload(args) -> dict:
+    """The headline step's assembly alone (cell records + residual + BSR Jacobian)
+    on the phase-updated state, timed over --steps evaluations: the launches a
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE pass of the assembly kernel covers."""
+    import torch
+
+    fluid, fl, asm, s, host = build_workload(args.nx, args.ny, args.nz, eos=args.eos)
+    run_step(fl, asm, s)
+    torch.cuda.synchronize()
+    fl.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        asm.evaluate(DT, s["prim"], s["ps"], s["acc"], out=s["out"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms, count = fl.kernel_time("assemble")
+    rms, rcount = fl.kernel_time("cell_record")
+    fl.set_kernel_timing(False)
+    nc = fluid.nc
+    abytes = assemble_bytes(asm.ncell, asm.nconn, asm.nnzb, nc, asm.np_, asm.nps)
+    avg = ms / max(count, 1) * 1e-3
+    log(f"{args.steps} assemblies: assemble {ms / max(count, 1):.2f} ms, cell_record "
+        f"{rms / max(rcount, 1):.2f} ms per launch")
+    return {"metric": "Natural assembly evaluations/s", "value": args.steps / elapsed,
+            "unit": "assemblies/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+            "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+            "config": {"workload": f"assembly of the {args.nx}x{args.ny}x{args.nz} "
+                                   f"{EOS_NAMES[args.eos]} step", "cells": asm.ncell},
+            "kernels_ms": {"assemble": ms / max(count, 1), "cell_record": rms / max(rcount, 1)},
+            "roofline": {"kernel": "assembleKernel<3,true>", "bound": "hbm",
+                         "achieved": abytes / avg / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": abytes / avg / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes": abytes},
+            "cpu_baseline": None}
+
+
+def copy_workload(args) -> dict:
+    """Device-to-device copy of 2 GiB, three times: the calibration of the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE per dispatch against 2 GiB read and
+    2 GiB written)."""
+    import torch
+
+    n = (2 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    return {"metric": "device copy bandwidth", "value": 2 * (2 << 30) / dt / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 0, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+            "config": {"workload": "2 GiB device-to-device copy (counter calibration)",
+                       "bytes_read": 2 << 30, "bytes_written": 2 << 30},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -591,8 +723,10 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=50)
     ap.add_argument("--eos", choices=("CPA", "PR", "SW"), default="CPA")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("assembly", "flash", "learned", "serving", "cpa_phase"),
+    ap.add_argument("--workload", choices=("assembly", "flash", "learned", "serving", "cpa_phase",
+                                           "phase_cell_ops", "assembly_only", "copy"),
                     default="assembly")
+    ap.add_argument("--samples", type=int, default=256)
     ap.add_argument("--tune-sample", type=int, default=4096)
     ap.add_argument("--members", type=int, default=48)
     ap.add_argument("--max-member-cells", type=int, default=32768)
@@ -617,7 +751,10 @@ def main() -> int:
         return 1
     if args.workload != "assembly":
         line = {"flash": flash_workload, "learned": learned_workload,
-                "serving": serving_workload, "cpa_phase": cpa_phase_workload}[args.workload](args)
+                "serving": serving_workload, "cpa_phase": cpa_phase_workload,
+                "phase_cell_ops": phase_cell_ops_workload,
+                "assembly_only": assembly_only_workload,
+                "copy": copy_workload}[args.workload](args)
         print(json.dumps(line), flush=True)
         return 0
     fluid, fl, asm, s, host = build_workload(args.nx, args.ny, args.nz, eos=args.eos)
@@ -657,9 +794,27 @@ def main() -> int:
     abytes = assemble_bytes(ncell, nconn, nnzb, nc, np_, nps)
     avg_s = kern["assemble"]["avg_ms"] * 1e-3
     achieved = abytes / avg_s / 1e9
-    roofline = {"kernel": "assembleKernel<3,true>", "bound": "hbm", "achieved": achieved,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None, "algorithmic_bytes": abytes}
+    roofline_assembly = {"kernel": "assembleKernel<3,true>", "bound": "hbm",
+                         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": ASSEMBLE_TRAFFIC_BYTES,
+                         "algorithmic_bytes": abytes}
+    # the dominant kernels: the phase update's passes (record, lockstep trials, replay,
+    # re-flash; cooperative), FP64 work per cell x cells / their measured time
+    pu_s = (kern["phase_update"]["avg_ms"] + kern["phase_update_coop"]["avg_ms"]) * 1e-3
+    eos_cpa = args.eos == "CPA"
+    flops = PHASE_UPDATE_FP64_OPS_PER_CELL * ncell if eos_cpa else None
+    tflops = flops / pu_s / 1e12 if eos_cpa else None
+    roofline = {"kernel": "phase update: phaseUpdateKernel + stabilityTrialKernel + "
+                          "phaseUpdateReplayKernel + phaseUpdateReflashKernel + "
+                          "phaseUpdateCoopKernel",
+                "bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS if eos_cpa else None,
+                "traffic": PHASE_UPDATE_TRAFFIC_BYTES if eos_cpa else None,
+                "algorithmic_flops": flops,
+                "note": "FP64 vector (VALU) work; MI355X's FP64 vector and matrix peaks are both "
+                        "78.6 TFLOP/s and no MFMA is used. Work per cell measured by FP64 "
+                        "instruction counters on single-cell launches (PR-CPA only); traffic "
+                        "is L2<->memory bytes of the passes (PMC), mostly per-lane scratch"}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -685,6 +840,7 @@ def main() -> int:
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "kernels_ms": {k: v["avg_ms"] for k, v in kern.items()},
         "roofline": roofline,
+        "roofline_assembly": roofline_assembly,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

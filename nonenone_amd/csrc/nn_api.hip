@@ -297,6 +297,15 @@ hipError_t reflashSort(void *arg, hipStream_t st, const unsigned int **order)
     return sortOrder(a->ctx, a->n, st, order);
 }
 
+// Ordered batches sort with hipCUB, whose radix sort issues hipMemsetAsync; a
+// memset recorded into a captured graph is not replayed reliably here, so a
+// batch issued on a capturing stream runs in index order.
+bool streamCapturing(hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+}
+
 unsigned int coopGrid(const nn_context *ctx)
 {
     return static_cast<unsigned int>(ctx->coopBlocks > 0
@@ -707,7 +716,8 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
     }
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
-    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems && n <= kOrderMaxItems;
+    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems && n <= kOrderMaxItems &&
+                         !streamCapturing(st);
     if (ordered)
         if (const int rc = ensureSort(ctx, n); rc != NN_OK)
             return rc;
@@ -947,11 +957,12 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
-    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems && n <= kOrderMaxItems;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems && n <= kOrderMaxItems &&
+                         !streamCapturing(st);
     if (ordered)
         if (const int rc = ensureSort(ctx, n); rc != NN_OK)
             return rc;
-    hipStream_t st = static_cast<hipStream_t>(stream);
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
         resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue, ctx->coopAll ? 1u : 0u,
