@@ -579,8 +579,131 @@ __host__ __device__ S cpaPressure(const FluidParams &f, const CpaTemp<NC> &c, co
     return cubic + association;
 }
 
-// :2090-2178 (scalar density root: bracket, then up to 100 bisections)
+// Certified outcome of one bracket or bisection evaluation of the density root
+// for the single-donor scheme, without the exact site solve.
+//
+// The exact evaluation r(rho) = cpaPressure(rho) - P solves the donor site
+// balance h(X) = X (1 + sum_j aC_j / (1 + bC_j X)) - 1 = 0 by Newton from
+// 1 / (1 + sum aC) until the computed |h| <= cpaTol.  With aC, bC >= 0,
+// h'(X) = 1 + sum_j aC_j / (1 + bC_j X)^2 >= 1 for X >= 0, so any X lies within
+// |h(X)| of the root: the exact solver's X and a warm-started X_a differ by at
+// most cpaTol + |h(X_a)| plus the rounding of both computed residuals.  The
+// pressure depends on X only through the bonded-site sum
+//   x_d n_d (1 - X) + sum_j x_j n_j (1 - 1 / (1 + bC_j X)),
+// whose X-derivative is bounded by x_d n_d + sum_j x_j n_j bC_j; that, times
+// R T rho (1 + rho b g'/(4 g)) / 2, plus a rounding allowance of 64 ulps on every
+// term, bounds |r_exact - r_a| by B/2.  If |r_a| > B + stop, the exact
+// evaluation has r_a's sign and does not meet the bisection's |r| <= stop test:
+// the search takes the same branch without it.
+//
+// Returns +1 or -1 (certified sign) or 0 (evaluate exactly).  X carries the
+// warm start from one call to the next (outside (0, 1]: none yet).  The
+// constants of one root search are taken once (CpaSkip); products regrouped
+// against the exact evaluation differ from it by a few ulps, inside the bound's
+// 64-ulp allowances.
 template <int NC>
+struct CpaSkip
+{
+    int donor = -1;   // the donor-bearing component, -1: no certified skips
+    double xd, nd;    // its mole fraction and donor-site count
+    double RT;
+    double na[NC];    // acceptor-site counts (0: not an acceptor of the donor)
+    double kDelta[NC];// Delta_dj / g
+    double slopeBound;// without the bC_j terms (x_d n_d)
+    double sites;     // sum_j x_j (donor + acceptor sites)
+};
+
+template <int NC>
+__host__ __device__ void cpaSkipSetup(const FluidParams &f, const CpaTemp<NC> &c, double T,
+                                      const double *x, int donor, CpaSkip<NC> &k)
+{
+    k.donor = donor;
+    k.xd = x[donor];
+    k.nd = double(f.cpaDonors[donor]);
+    k.RT = kGasConstant * T;
+    k.slopeBound = k.xd * k.nd;
+    k.sites = 0.0;
+    for (int j = 0; j < NC; ++j)
+    {
+        k.sites += x[j] * double(f.cpaDonors[j] + f.cpaAcceptors[j]);
+        k.na[j] = f.cpaAcceptors[j] > 0 ? double(f.cpaAcceptors[j]) : 0.0;
+        k.kDelta[j] = cpaDelta<NC, double>(c, donor, j, 1.0);
+        if (!(k.kDelta[j] > 0.0))
+            k.na[j] = 0.0;
+    }
+}
+
+template <int NC>
+__host__ __device__ int cpaCertifiedSign(const FluidParams &f, double rho, const double *x,
+                                         const CpaMix<NC, double> &m, double P, double stop,
+                                         const CpaSkip<NC> &k, double &X)
+{
+    constexpr double ulp64 = 64.0 * 1.1102230246251565e-16;
+    const double q = m.b * rho;
+    double attraction = 1.0 + q;
+    if (f.cpaTerm == 1)
+    {
+        constexpr double sqrtTwo = 1.4142135623730950488;
+        attraction = (1.0 + (1.0 + sqrtTwo) * q) * (1.0 + (1.0 - sqrtTwo) * q);
+    }
+    const double repulsive = k.RT * rho / (1.0 - q);
+    const double attractive = m.a * rho * rho / attraction;
+    const double cubic = repulsive - attractive;
+    const double g = 1.0 / (1.0 - 1.9 * (m.b * rho / 4.0));
+    double aC[NC], bC[NC];
+    double aSum = 0.0, slopeBound = k.slopeBound;
+    for (int j = 0; j < NC; ++j)
+    {
+        const double delta = g * k.kDelta[j];
+        aC[j] = rho * x[j] * k.na[j] * delta;
+        bC[j] = rho * k.xd * k.nd * delta;
+        aSum += aC[j];
+        slopeBound += x[j] * k.na[j] * bC[j];
+    }
+    if (!(X > 0.0 && X <= 1.0))
+        X = 1.0 / (1.0 + aSum);
+    const double scale = 0.5 * k.RT * rho * (1.0 + 1.9 * g * m.b * rho / 4.0);
+    const double roundCubic = ulp64 * (::fabs(repulsive) + ::fabs(attractive) + ::fabs(P));
+    for (int step = 0; step < 3; ++step)
+    {
+        double sum = 0.0, slope = 0.0, bonded = k.xd * k.nd * (1.0 - X);
+        for (int j = 0; j < NC; ++j)
+        {
+            if (k.na[j] == 0.0)
+                continue;
+            const double inv = 1.0 / (1.0 + bC[j] * X);
+            const double t = aC[j] * inv;
+            sum += t;
+            slope -= t * bC[j] * inv;
+            bonded += x[j] * k.na[j] * (1.0 - inv);
+        }
+        const double h = X * (1.0 + sum) - 1.0;
+        const double association = -scale * bonded;
+        const double r = cubic + association - P;
+        const double hRound = ulp64 * (1.0 + X * (1.0 + sum));
+        const double dX = f.cpaTol + ::fabs(h) + 2.0 * hRound;
+        const double bound = 2.0 * (scale * (slopeBound * dX + ulp64 * k.sites) + roundCubic +
+                                    ulp64 * ::fabs(association));
+        if (!finite(r) || !finite(bound))
+            return 0;
+        if (::fabs(r) > bound + stop)
+            return r > 0.0 ? 1 : -1;
+        const double next = X - h / (1.0 + sum + X * slope);
+        if (!(next > 0.0 && next <= 1.0))
+            return 0;
+        X = next;
+    }
+    return 0;
+}
+
+#ifndef NN_CPA_CERTIFIED_SKIP
+#define NN_CPA_CERTIFIED_SKIP 1
+#endif
+
+// :2090-2178 (scalar density root: bracket, then up to 100 bisections).  With
+// Skip, evaluations whose branch cpaCertifiedSign settles are not made; the
+// bracket, the midpoints and the returned root are those of the plain search.
+template <int NC, bool Skip = (NN_CPA_CERTIFIED_SKIP != 0)>
 __host__ __device__ double cpaDensityRootValue(const FluidParams &f, const CpaTemp<NC> &c,
                                                double P, double T, const double *x, bool liquid,
                                                const CpaMix<NC, double> &m, Err &err)
@@ -592,7 +715,29 @@ __host__ __device__ double cpaDensityRootValue(const FluidParams &f, const CpaTe
         return 0.0;
     }
     const double rhoMax = 0.999999 / b;
+    const double stop = 1.0e-10 * maxd(P, 1.0);
+    CpaSkip<NC> skip;
+    if constexpr (Skip)
+    {
+        // single donor-bearing component, a site solver that cannot run out of
+        // iterations, non-negative composition (the bound needs aC, bC >= 0)
+        const int scheme = cpaScheme(f);
+        bool ok = scheme <= -2 && scheme != kCpaGeneral && f.cpaMaxIter >= 32 && f.cpaTol > 0.0;
+        for (int i = 0; i < NC; ++i)
+            ok = ok && x[i] >= 0.0;
+        if (ok)
+            cpaSkipSetup<NC>(f, c, T, x, -2 - scheme, skip);
+    }
+    double warmX = -1.0;
+    // The residual, or +-inf where its sign is certified and |residual| > stop
+    // (every use below reads only its sign and that test).
     auto residual = [&](double rho) {
+        if (skip.donor >= 0)
+        {
+            const int s = cpaCertifiedSign<NC>(f, rho, x, m, P, stop, skip, warmX);
+            if (s != 0)
+                return s > 0 ? __builtin_inf() : -__builtin_inf();
+        }
         return cpaPressure<NC, double>(f, c, rho, T, x, m, err) - P;
     };
     double lo = 0.0, hi = 0.0, flo = -P, fhi = 0.0;

@@ -95,6 +95,19 @@ struct ParamCache
     MixParams<NC, double> nonAqueous;
     MixParams<NC, double> aqueous;
     CpaTemp<NC> cpa;  // CPA: temperature part of the mixing state (cubic_eos.hpp:1356-1414)
+    // CPA: the last two phase evaluations (inputs and results).  A CPA phase
+    // evaluation is a pure function of (P, T, x, liquid) once the temperature
+    // part is cached, and the flash's loops often repeat the previous one or
+    // the one before (e.g. the converged phases again for the canonical
+    // thermodynamics or a stability test's reference phase), so a repeated call
+    // takes the stored result: the same bits, without the density root.
+    struct CpaMemo
+    {
+        double P = -1.0, T = -1.0, x[NC] = {};
+        bool liquid = false;
+        double Z = 0.0, phi[NC] = {};
+    } memo[2];
+    int memoNext = 0;
 };
 
 // Gaussian elimination with partial pivoting (three_phase_flash_stability.inc:98-146).
@@ -220,10 +233,36 @@ public:
             // three_phase_flash_core.inc:108-112: the EOS's own role-aware CPA result
             if (cache.cpa.T != T)
                 cpaTemperature<NC>(f, T, cache.cpa);
+            const bool liquid = role != kGas;
+            for (int k = 0; k < 2; ++k)
+            {
+                const auto &m = cache.memo[k];
+                bool same = m.P == P && m.T == T && m.liquid == liquid;
+                for (int i = 0; i < NC; ++i) same = same && m.x[i] == x[i];
+                if (same)
+                {
+                    out.Z = m.Z;
+                    for (int i = 0; i < NC; ++i) out.phi[i] = m.phi[i];
+                    return;
+                }
+            }
             double lnPhi[NC];
-            cpaPhase<NC, double, Mode == kModeCoop>(f, cache.cpa, P, T, x, role != kGas, out.Z, lnPhi,
-                                                    err);
+            cpaPhase<NC, double, Mode == kModeCoop>(f, cache.cpa, P, T, x, liquid, out.Z, lnPhi, err);
             for (int i = 0; i < NC; ++i) out.phi[i] = err.bad() ? 1.0 : ::exp(lnPhi[i]);
+            if (!err.bad())
+            {
+                auto &m = cache.memo[cache.memoNext];
+                cache.memoNext ^= 1;
+                m.P = P;
+                m.T = T;
+                m.liquid = liquid;
+                m.Z = out.Z;
+                for (int i = 0; i < NC; ++i)
+                {
+                    m.x[i] = x[i];
+                    m.phi[i] = out.phi[i];
+                }
+            }
             return;
         }
         ensureCache(P, T);

@@ -100,3 +100,37 @@ def test_oracle_eos_matches_reference_build(model):
     assert np.array_equal(so, sr)
     np.testing.assert_allclose(Zo, Zr, rtol=1e-13, atol=0)
     np.testing.assert_allclose(lo, lr, rtol=1e-11, atol=1e-13)
+
+
+# Listed (libm-conditioned) states per test set, as tests/golden/make_sensitivity.py
+# wrote them; a regenerated list that grows past these counts fails here.
+CONDITIONED_CEILING = {"flash_nc3_cpa": 24, "flash_nc3_sw": 45, "flash_nc6_sw": 47,
+                       "live_nc3_cpa": 31, "live_nc3_sw": 54, "live_nc6_sw": 69,
+                       "update_sw6_60x20_seed21": 38, "update_sw_60x20_seed21": 12}
+
+
+def test_every_libm_conditioned_state_has_a_finite_bound():
+    import json
+
+    import libm_conditioning as lc
+
+    sets = json.loads(lc.PATH.read_text())["sets"]
+    assert set(sets) == set(CONDITIONED_CEILING)
+    for name, entry in sets.items():
+        n = entry["n"]
+        listed = lc.entries(name)
+        assert len(listed) <= CONDITIONED_CEILING[name], name
+        lim = lc.bounds(name, n, 1e-10)
+        assert np.all(np.isfinite(lim)) and lim.shape == (n,)
+        for i, e in listed.items():
+            expect = max(1e-10, lc.BAND_FACTOR * e["band"])
+            if e["gibbs_tie"] or e["allocation_recovery"]:
+                expect = max(expect, lc.TIE_BOUND)
+            assert lim[i] == expect
+        unlisted = np.setdiff1d(np.arange(n), list(listed))
+        assert np.all(lim[unlisted] == 1e-10)
+        # a state over its bound is a failure, one within it is only reported
+        per = lim.copy()
+        per[unlisted[:1]] *= 2.0
+        failures, _ = lc.split_value_failures(per, np.ones(n, bool), 1e-10, name)
+        assert failures == [int(unlisted[0])]
