@@ -36,7 +36,8 @@ Other workloads (same JSON contract, for the figures quoted beside the headline)
                         of states within 2 % of a Kay's-rule pseudo-critical point),
                         with launch shapes picked by the autotuner on an 8192-state
                         sample of each sub-batch;
-                        also times the plain one-kernel form on the same states.
+                        also times the plain form (one work-item per state,
+                        index order) on the same states.
     --workload serving  64 ensemble members (nc=3/5/6/8, PR/SW/PR-CPA, 1k-32k cells each)
                         submit 256 Newton-step requests with Poisson arrivals (200/s);
                         continuous batching (phase updates of members sharing a
@@ -81,9 +82,9 @@ FP64_PEAK_TFLOPS = 78.6
 # FP64 operations of one cell's phase update on the PR-CPA headline workload
 # (add, mul, transcendental = 1, fma = 2), measured by the FP64 instruction
 # counters of `bench.py --workload phase_cell_ops` (256 seeded cells, one cell
-# per launch sequence, so one active lane per wavefront): 3.1103e9 over 256 cells.
+# per launch sequence, so one active lane per wavefront): 1.5349e9 over 256 cells.
 # profiles/phase_update_cpa_fp64_ops_single_cell_sample.md
-PHASE_UPDATE_FP64_OPS_PER_CELL = 1.2150e7
+PHASE_UPDATE_FP64_OPS_PER_CELL = 5.9956e6
 # HBM bytes (L2 <-> memory, FETCH_SIZE x 2 + WRITE_SIZE x 1 against a 2 GiB copy)
 # of one headline step's kernels, from the two PMC passes of
 # profiles/phase_update_natural_assembly_200x200x50_cpa_hbm_traffic.md
@@ -313,13 +314,11 @@ def flash_workload(args) -> dict:
     nonok = sum(int((sb["out"].status != 0).sum().item()) for sb in subs)
     log(f"timed {args.steps} steps in {elapsed:.3f}s; non-OK statuses {nonok}")
 
-    # the plain one-kernel form on the same states (one pass)
+    # the plain form (one work-item per state, index order) on the same states (one pass)
     plain = {}
     t_plain = 0.0
     for sb in subs:
         fl = sb["fl"]
-        if fl.nc != 3:
-            continue   # the plain form is built for three components
         keep = fl.schedule()
         fl.set_schedule(plain=1)
         ref_out = FlashBatchResult.empty(per, fl.nc)
@@ -334,7 +333,7 @@ def flash_workload(args) -> dict:
                    for f in ("presence", "status", "beta", "x", "Z"))
         plain["/".join(sb["case"])] = dict(seconds=dt_, bit_identical=same)
         log(f"plain form {sb['case']}: {dt_:.3f}s, results bit-identical: {same}")
-    t_sched3 = sum(sb.get("seconds", 0.0) for sb in subs if sb["fl"].nc == 3)
+    t_sched3 = sum(sb.get("seconds", 0.0) for sb in subs)
     per_step = elapsed / args.steps
     abytes = sum(per * flash_bytes(sb["fl"].nc) for sb in subs)
     achieved = abytes / per_step / 1e9
@@ -354,8 +353,8 @@ def flash_workload(args) -> dict:
                                "near-critical", "states": total},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "kernels_ms": kern,
-        "plain_form": {"cases": "nc=3 sub-batches (the plain form is built for three "
-                                "components)", "seconds": t_plain,
+        "plain_form": {"cases": "every sub-batch, one work-item per state in index order",
+                       "seconds": t_plain,
                        "scheduled_seconds": t_sched3,
                        "speedup_of_scheduled": t_plain / t_sched3 if t_sched3 > 0 else None,
                        "per_case": plain},
@@ -766,6 +765,11 @@ def main() -> int:
         log(f"warmup {i}: {time.time() - t0:.3f}s; phase-update non-OK "
             f"{int((st != 0).sum().item())}; assembly non-OK "
             f"{int((s['out']['status'] != 0).sum().item())}")
+    queue = None
+    if args.warmup > 0:
+        queue = fl.queue_counts()
+        log(f"phase-update cells: {queue['recorded']} recorded for the lockstep trials, "
+            f"{queue['reflashed']} re-flashed, {queue['deferred']} on the wavefront-per-cell pass")
 
     fl.set_kernel_timing(True)
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -839,6 +843,7 @@ def main() -> int:
                    "connections": nconn, "bsr_blocks": nnzb, "n_primary": np_},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "kernels_ms": {k: v["avg_ms"] for k, v in kern.items()},
+        "phase_update_cells": queue,
         "roofline": roofline,
         "roofline_assembly": roofline_assembly,
         "cpu_baseline": cpu,

@@ -84,7 +84,32 @@ typedef struct nn_fluid_desc {
     double cpa_cross_beta[NN_MAX_NC * NN_MAX_NC];
     /* Corey relative permeability: swc, sorw, sgc, sorg, krw_max, krg_max, kro_max */
     double relperm[7];
+    /* Property closures of the Water role (all zero = off; case_config.hpp:70-74
+     * switches both on in the reference's benchmark case):
+     *   aqueous_volume 1: IAPWS-IF97 water density + Garcia (2001) CO2 apparent
+     *     volume for the Water-role molar volume (CubicEquationOfState::
+     *     configureAqueousVolume, cubic_eos.hpp:405-412; aqueous_volume.hpp),
+     *     with its composition domain: at most aqueous_volume_max_unsupported of
+     *     components other than H2O and CO2;
+     *   aqueous_viscosity NN_AQ_VISCOSITY_MCBRIDE_WRIGHT: McBride-Wright (2015)
+     *     H2O-CO2 viscosity (FluidSystem::configureMcBrideWrightAqueousViscosity),
+     *     NN_AQ_VISCOSITY_IAPWS2008: IAPWS-2008 industrial pure-water viscosity
+     *     (configureIapws2008AqueousViscosity together with the EOS composition
+     *     domain configureAqueousCompositionDomain); aqueous_viscosity_max_unsupported
+     *     is the former's non-H2O/CO2 and the latter's non-H2O limit
+     *     (aqueous_viscosity.hpp). */
+    int32_t aqueous_volume;
+    int32_t aqueous_viscosity;
+    int32_t co2_component;
+    int32_t aqueous_reserved;
+    double water_molar_mass;                  /* kg/mol, e.g. 0.018015268 */
+    double aqueous_volume_max_unsupported;    /* e.g. 1e-4 */
+    double aqueous_viscosity_max_unsupported; /* e.g. 1e-4 */
 } nn_fluid_desc;
+
+#define NN_AQ_VISCOSITY_NONE 0
+#define NN_AQ_VISCOSITY_MCBRIDE_WRIGHT 1
+#define NN_AQ_VISCOSITY_IAPWS2008 2
 
 /* ThreePhaseFlashOptions (three_phase_flash_types.hpp:21-41); waterComponent
  * is taken from nn_fluid_desc.water_component. */
@@ -142,6 +167,13 @@ const char *nn_last_error(void);
 int nn_context_set_kernel_timing(nn_context *ctx, int32_t enable);
 int nn_context_kernel_time(nn_context *ctx, int32_t kind, double *total_ms,
                            int64_t *launches);
+
+/* Work counters of the context's last flash or phase-update launch (no
+ * counterpart in the reference: instrumentation; synchronises the device):
+ * items handed to the wavefront-per-item pass, cells recorded for the lockstep
+ * stability trials, and cells held for the re-flash pass. */
+int nn_context_queue_counts(nn_context *ctx, int64_t *deferred, int64_t *recorded,
+                            int64_t *reflashed);
 
 /*
  * E1-E5: batched single-phase EOS evaluation, CubicEquationOfState::phaseResult
@@ -229,9 +261,10 @@ int nn_flash_batch_static(nn_context *ctx, int64_t n,
  *                             condition-key order
  *   wave_claim                0: lanes claim single states from the queue;
  *                             1: wavefronts claim tiles of 64 consecutive states
- *   plain                     1: the baseline form for flash batches: one kernel,
- *                             one work-item per state in index order, no queue,
- *                             no deferral (other fields ignored)
+ *   plain                     1: the baseline form for flash batches: one
+ *                             work-item per state in index order (no persistent
+ *                             queue, no sort, other fields ignored); states past
+ *                             the lane budget still go to the cooperative pass
  */
 typedef struct nn_schedule {
     int32_t persistent_blocks_per_cu;
@@ -356,8 +389,8 @@ typedef struct nn_assembly nn_assembly;
 
 /* Uploads topology (device pointers, kept by reference, must outlive the
  * handle) and precomputes, per connection, the BSR slot of its reciprocal block
- * and, per cell, the slot of its diagonal block.  Synchronous.  Built for fluids
- * of at most 6 components (NN_E_UNSUPPORTED above). */
+ * and, per cell, the slot of its diagonal block.  Synchronous.  Any component
+ * count up to NN_MAX_NC. */
 int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
                        const int64_t *d_conn_offsets, const int64_t *d_conn_neighbor,
                        const double *d_conn_trans, const double *d_conn_gdz,
@@ -419,6 +452,25 @@ typedef struct nn_scaling {
 void nn_scaling_default(nn_scaling *s);
 int nn_assembly_set_scaling(nn_assembly *a, const nn_scaling *s, const double *d_well_row_scale);
 
+/* Newton-step limiter of the Natural runtime, in place on the device
+ * (replaces NaturalPetscRuntime::limitNewtonStep, natural_petsc_runtime_state.inc:
+ * 452-545, with state/newton_limiter.hpp:124-245).  d_step holds PETSc's
+ * correction Y for every cell (ncell x np, the primary layout), d_primary the
+ * current solution.  Per cell, on delta = -Y: with the handle's scaling
+ * enabled (nn_assembly_set_scaling) the scaled increment is recovered
+ * (delta[q] *= variable scale q); the three saturation increments are limited
+ * together to |dS| <= 0.1, each phase's composition block (its independent
+ * fractions and the dependent last component) to 0.1, the pressure to 25 % of
+ * |p|; with max_aqueous_unsupported >= 0 the Water composition block is then
+ * scaled so that the Water role's mole fraction of components other than
+ * water_component and co2_component stays below that ceiling
+ * (limitAqueousCompositionNewtonIncrement, maximumAqueousUnsupportedNewton
+ * MoleFraction of natural_runtime_options.hpp:49).  d_step receives -delta.
+ * Asynchronous on `stream`. */
+int nn_newton_limit(nn_assembly *a, const double *d_primary, double *d_step,
+                    double max_aqueous_unsupported, int32_t water_component,
+                    int32_t co2_component, void *stream);
+
 /* Sizes of an assembly handle: its context, cells, BSR blocks and primary
  * variables per cell (no counterpart in the reference). */
 int nn_assembly_info(const nn_assembly *a, nn_context **ctx, int64_t *ncell,
@@ -439,8 +491,13 @@ int nn_assembly_info(const nn_assembly *a, nn_context **ctx, int64_t *ncell,
  * arrive while a batch runs join the next one.  Everything is issued on one
  * stream (the one given at creation, NULL = default): kernels whose lanes need
  * tens of KB of scratch could not be given scratch on additional hardware
- * queues, and merged launches fill the GPU without them.  The phase update is
- * per cell, so results are the same alone and in any mix.  Steps of one member
+ * queues, and merged launches fill the GPU without them.  The phase update's
+ * arithmetic is per cell; results are bit-identical alone and in the mix in the
+ * tested cases (members of up to a few thousand cells), but in large merged
+ * launches some five- and six-component steps have come out differently from
+ * the one-step form (the device build's results have depended on which cells
+ * share a wavefront, cause not identified), so bitwise order independence is
+ * not promised there.  Steps of one member
  * run in submission order.  Output buffers belong to the step until
  * nn_server_release.  max_batch_steps = 1 gives the plain one-request-at-a-time
  * form.  fluid_group: steps whose members share fluid and flash options may

@@ -69,6 +69,13 @@ class FluidDesc(ctypes.Structure):
         ("cpa_cross_epsilon", _D * (NN_MAX_NC * NN_MAX_NC)),
         ("cpa_cross_beta", _D * (NN_MAX_NC * NN_MAX_NC)),
         ("relperm", _D * 7),
+        ("aqueous_volume", _I32),
+        ("aqueous_viscosity", _I32),
+        ("co2_component", _I32),
+        ("aqueous_reserved", _I32),
+        ("water_molar_mass", _D),
+        ("aqueous_volume_max_unsupported", _D),
+        ("aqueous_viscosity_max_unsupported", _D),
     ]
 
 
@@ -227,6 +234,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.nn_assembly_evaluate.argtypes = [_P, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P]
     L.nn_context_set_kernel_timing.argtypes = [_P, _I32]
     L.nn_context_kernel_time.argtypes = [_P, _I32, ctypes.POINTER(_D), ctypes.POINTER(_I64)]
+    L.nn_context_queue_counts.argtypes = [_P] + [ctypes.POINTER(_I64)] * 3
     L.nn_flash_seeded_batch.argtypes = [_P, _I64, _P, _P, _P, _P, _P, ctypes.POINTER(FlashOut),
                                         _P]
     L.nn_context_get_schedule.argtypes = [_P, ctypes.POINTER(Schedule)]
@@ -258,7 +266,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.nn_assembly_info.argtypes = [_P, ctypes.POINTER(_P), ctypes.POINTER(_I64),
                                    ctypes.POINTER(_I64), ctypes.POINTER(_I32)]
     L.nn_assembly_info.restype = ctypes.c_int
-    for name in ("nn_context_set_kernel_timing", "nn_context_kernel_time","nn_context_create", "nn_context_destroy", "nn_eos_phase_batch",
+    L.nn_newton_limit.argtypes = [_P, _P, _P, _D, _I32, _I32, _P]
+    L.nn_newton_limit.restype = ctypes.c_int
+    for name in ("nn_context_set_kernel_timing", "nn_context_kernel_time",
+                 "nn_context_queue_counts", "nn_context_create", "nn_context_destroy", "nn_eos_phase_batch",
                  "nn_flash_batch", "nn_flash_restricted_batch", "nn_stability_batch",
                  "nn_phase_update_batch", "nn_structured_topology", "nn_assembly_create",
                  "nn_assembly_destroy", "nn_assembly_accumulation", "nn_assembly_evaluate",
@@ -285,8 +296,14 @@ EXPORTED_SYMBOLS = (
     "nn_server_create", "nn_server_destroy", "nn_server_submit_step", "nn_server_wait",
     "nn_server_release", "nn_server_get_stats", "nn_server_last_error", "nn_server_hold",
     "nn_flash_batch_static", "nn_scaling_default", "nn_assembly_set_scaling",
-    "nn_phase_update_transaction", "nn_bsr_cell_columns",
+    "nn_phase_update_transaction", "nn_bsr_cell_columns", "nn_context_queue_counts",
+    "nn_newton_limit",
 )
+
+# Water-role viscosity closures (nn_fluid_desc.aqueous_viscosity).
+NN_AQ_VISCOSITY_NONE = 0
+NN_AQ_VISCOSITY_MCBRIDE_WRIGHT = 1
+NN_AQ_VISCOSITY_IAPWS2008 = 2
 
 # Kernel kinds of nn_context_kernel_time.
 KERNEL_KINDS = {"eos": 0, "flash": 1, "stability": 2, "phase_update": 3, "cell_record": 4,

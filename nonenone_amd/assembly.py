@@ -79,6 +79,20 @@ class NaturalAssembly:
                                                     _ptr(self._well_row_scale)),
                    "nn_assembly_set_scaling")
 
+    def limit_newton_step(self, primary: torch.Tensor, step: torch.Tensor,
+                          max_aqueous_unsupported: float | None = None, water: int = 0,
+                          co2: int = 1) -> torch.Tensor:
+        """NaturalPetscRuntime::limitNewtonStep on the device, in place on `step`
+        (PETSc's correction Y, ncell x np): scaling recovery when the assembly's
+        scaling is on, the phase-aware increment limiter, and with
+        `max_aqueous_unsupported` the Water composition block kept inside the
+        aqueous closures' domain (natural_petsc_runtime_state.inc:452-545)."""
+        limit = -1.0 if max_aqueous_unsupported is None else float(max_aqueous_unsupported)
+        _abi.check(self.lib.nn_newton_limit(self.handle, _ptr(primary), _ptr(step), limit,
+                                            int(water), int(co2), _stream()),
+                   "nn_newton_limit")
+        return step
+
     def accumulation(self, primary: torch.Tensor, phase_state: torch.Tensor) -> torch.Tensor:
         acc = torch.empty(self.ncell, self.nc, dtype=torch.float64, device=primary.device)
         _abi.check(self.lib.nn_assembly_accumulation(self.handle, _ptr(primary), _ptr(phase_state),

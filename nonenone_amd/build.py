@@ -27,8 +27,16 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = Path(os.environ.get("NN_LIB_OUT", str(HERE / "libnonenone_amd.so")))
 BUILD_DIR = Path(os.environ.get("NN_BUILD_DIR", str(HERE.parent / "_build")))
-# Minimum wavefronts per SIMD of the per-state kernels (csrc/launch.hpp).
+# Minimum wavefronts per SIMD of the per-state kernels (csrc/launch.hpp).  Only 1
+# is built: a build for 2 waves per SIMD (256-VGPR budget, 15.8 KB of private
+# segment per lane in flashKernel<3>) stopped on the GPU with a memory-aperture
+# violation in its first 262,144-state flash batch, cause not identified.
 WAVES_PER_EU = os.environ.get("NN_WAVES_PER_EU")
+if WAVES_PER_EU not in (None, "", "1"):
+    raise RuntimeError("NN_WAVES_PER_EU: only 1 wavefront per SIMD is built (the 2-wave build "
+                       "faulted on the GPU with a memory-aperture violation)")
+# Derivative columns per chunk of the assembly kernel's AD (csrc/launch.hpp).
+ASSEMBLY_CHUNK = os.environ.get("NN_ASSEMBLY_CHUNK")
 ARCH = os.environ.get("NN_OFFLOAD_ARCH", "gfx950")
 UNITS = ("nn_api.hip", "nn_nc2.hip", "nn_nc3.hip", "nn_nc4.hip", "nn_nc5.hip", "nn_nc6.hip", "nn_nc8.hip",
          "mlp.hip", "serve.hip")
@@ -53,11 +61,15 @@ def _filtered(stderr: str) -> str:
 def _compile(unit: str) -> tuple[str, Path, str]:
     obj = BUILD_DIR / (unit + ".o")
     extra = [f"-DNN_WAVES_PER_EU={int(WAVES_PER_EU)}"] if WAVES_PER_EU else []
+    if ASSEMBLY_CHUNK:
+        extra.append(f"-DNN_ASSEMBLY_CHUNK={int(ASSEMBLY_CHUNK)}")
     cmd = [hipcc(), *FLAGS, *extra, "-Rpass-analysis=kernel-resource-usage",
            "-I", str(HERE.parent / "include"), "-c", str(CSRC / unit), "-o", str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"hipcc failed on {unit} ({proc.returncode}):\n{_filtered(proc.stderr)}")
+    # per-kernel registers, scratch and occupancy, kept beside the object
+    (BUILD_DIR / (unit + ".resources.txt")).write_text(proc.stderr)
     # A kernel whose stack the compiler cannot bound (recursion, indirect calls)
     # would run with a runtime-default stack and overflow its scratch: refuse it.
     kernel = None

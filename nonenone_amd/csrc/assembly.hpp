@@ -21,6 +21,7 @@
 
 #include "common.hpp"
 #include "dual.hpp"
+#include "aqueous.hpp"
 #include "cpa.hpp"
 #include "eos.hpp"
 #include "phase_update.hpp"
@@ -50,15 +51,19 @@ struct CellPropsT
     S porosity;
 };
 
-// state_codec.hpp:99-176; `seed` = true creates the primary-variable seeds.
+// state_codec.hpp:99-176.  The derivative components of S seed primary
+// variables col0 .. col0 + S::size - 1 (a derivative chunk; col0 = 0 and
+// S::size = N_p seed them all).
 template <int NC, bool Well, class S>
-__host__ __device__ void decodeState(const double *prim, uint8_t presence, CellStateT<NC, S> &st)
+__host__ __device__ void decodeState(const double *prim, uint8_t presence, CellStateT<NC, S> &st,
+                                     int col0 = 0)
 {
     using L = Layout<NC, Well>;
     auto var = [&](int idx) {
         S r(prim[idx]);
         if constexpr (S::size > 0)
-            r.d[idx] = 1.0;
+            if (idx >= col0 && idx < col0 + S::size)
+                r.d[idx - col0] = 1.0;
         return r;
     };
     st.presence = presence;
@@ -148,12 +153,15 @@ __host__ __device__ S lbcViscosity(const FluidParams &f, const S *x, const S &rh
     return atmospheric + (poly4 + -1.0e-4) / epsMix;
 }
 
-// cell_property_evaluator.hpp:236-393 (PR / SW; no aqueous closures, no Land,
-// no pore-volume multiplier).
+// cell_property_evaluator.hpp:236-393 (PR / SW / PR-CPA, with the Water-role
+// closures when configured; no Land, no pore-volume multiplier).  roots (PR-CPA, may be null): 3 x 3 doubles, per
+// phase the scalar density root and its two correction slopes of an earlier
+// evaluation of the same cell values (root > 0), or slots for them (root <= 0).
 template <int NC, class S>
 __host__ __device__ void evaluateProperties(const FluidParams &f, const CellStateT<NC, S> &st,
                                             double porosity, double Tcell,
-                                            CellPropsT<NC, S> &pr, Err &err)
+                                            CellPropsT<NC, S> &pr, Err &err,
+                                            double *roots = nullptr)
 {
     pr.porosity = S(porosity);
     S comp[3][NC];
@@ -177,7 +185,8 @@ __host__ __device__ void evaluateProperties(const FluidParams &f, const CellStat
         S lnPhi[NC];
         if (cpa)
         {
-            cpaPhase<NC, S>(f, cpaT, st.p, T, comp[q], q != kGas, Z, lnPhi, err);
+            cpaPhase<NC, S>(f, cpaT, st.p, T, comp[q], q != kGas, Z, lnPhi, err,
+                            roots ? roots + 3 * q : nullptr);
             if (err.bad())
                 return;
         }
@@ -215,11 +224,24 @@ __host__ __device__ void evaluateProperties(const FluidParams &f, const CellStat
             return;
         }
         for (int c = 0; c < NC; ++c) pr.w[q][c] = mass[c] / totalMass;
-        const S rhoMolar = st.p / (kGasConstant * T * Z);
+        // fluid_system.hpp:97-133: the phase molar density (Water role: the
+        // IAPWS-Garcia volume when configured) gives the mass density; the
+        // viscosity is LBC at the cubic molar density, or the Water role's closure
+        const S cubicRhoMolar = st.p / (kGasConstant * T * Z);
+        const S rhoMolar = f.aqVolume ? phaseMolarDensity<NC, S>(f, st.p, T, comp[q], Z,
+                                                                 q == kWater, err)
+                                      : cubicRhoMolar;
+        if (err.bad())
+            return;
         S mixMolarMass = 0.0;
         for (int c = 0; c < NC; ++c) mixMolarMass += comp[q][c] * f.mw[c];
         pr.rho[q] = rhoMolar * mixMolarMass;
-        pr.mu[q] = lbcViscosity<NC, S>(f, comp[q], rhoMolar, T);
+        if (q == kWater && f.aqViscosity != NN_AQ_VISCOSITY_NONE)
+            pr.mu[q] = aqueousViscosity<NC, S>(f, st.p, T, comp[q], err);
+        else
+            pr.mu[q] = lbcViscosity<NC, S>(f, comp[q], cubicRhoMolar, T);
+        if (err.bad())
+            return;
     }
     S sgRelperm = st.sat[kGas];
     if (!(st.presence & kGasBit))

@@ -97,11 +97,21 @@ enum : int
 };
 constexpr int kDeferBudget = 20000;  // RR3 residual evaluations per state
 
-// Wavefront helpers for kModeCoop (the host pass sees single-lane stand-ins).
+// Wavefront helpers for kModeCoop.  The host build sees single-lane stand-ins,
+// or, with NN_HOST_WAVE_EMULATION (test tooling: tests/coop_emulation.cpp), a
+// 64-lane wavefront emulated by 64 host threads that meet at every wave operation.
+#if !defined(__HIP_DEVICE_COMPILE__) && defined(NN_HOST_WAVE_EMULATION)
+int nnHostWaveLane();
+unsigned long long nnHostWaveBallot(bool pred);
+double nnHostWaveShfl(double v, int src);
+int nnHostWaveShfl(int v, int src);
+#endif
 __host__ __device__ inline int waveLane()
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __lane_id();
+#elif defined(NN_HOST_WAVE_EMULATION)
+    return nnHostWaveLane();
 #else
     return 0;
 #endif
@@ -110,6 +120,8 @@ __host__ __device__ inline unsigned long long waveBallot(bool pred)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __ballot(pred);
+#elif defined(NN_HOST_WAVE_EMULATION)
+    return nnHostWaveBallot(pred);
 #else
     return pred ? 1ull : 0ull;
 #endif
@@ -118,6 +130,8 @@ __host__ __device__ inline double waveShfl(double v, int src)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __shfl(v, src);
+#elif defined(NN_HOST_WAVE_EMULATION)
+    return nnHostWaveShfl(v, src);
 #else
     (void)src;
     return v;
@@ -127,6 +141,8 @@ __host__ __device__ inline int waveShfl(int v, int src)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __shfl(v, src);
+#elif defined(NN_HOST_WAVE_EMULATION)
+    return nnHostWaveShfl(v, src);
 #else
     (void)src;
     return v;
@@ -161,6 +177,9 @@ struct FluidParams
     double cpaA0[kMaxNc], cpaB[kMaxNc], cpaC1[kMaxNc], cpaEps[kMaxNc], cpaBeta[kMaxNc];
     int cpaDonors[kMaxNc], cpaAcceptors[kMaxNc];
     double cpaCrossEps[kMaxNc][kMaxNc], cpaCrossBeta[kMaxNc][kMaxNc];
+    // Water-role closures (nn_fluid_desc.aqueous_*)
+    int aqVolume, aqViscosity, co2;
+    double waterMolarMass, aqVolumeMax, aqViscosityMax;
 };
 
 struct FlashOptions

@@ -986,17 +986,29 @@ __host__ __device__ double cpaDensityRootValueCoop(const FluidParams &f, const C
 
 // :2191-2248: scalar root, then two implicit-function corrections
 // rho -= (p(rho) - P) / (dp/drho)_FD, which also carry first derivatives.
+// rootSlot (may be null): {root, FD slope of correction 1, of correction 2}.
+// rootSlot[0] > 0 holds those of an earlier call with the same values of P, T
+// and x (reused: a later derivative chunk of the same cell); otherwise this
+// call stores its own there.
 template <int NC, class S, bool Coop = false>
 __host__ __device__ S cpaDensityRoot(const FluidParams &f, const CpaTemp<NC> &c, const S &P,
-                                     const S &T, const S *x, bool liquid, Err &err)
+                                     const S &T, const S *x, bool liquid, Err &err,
+                                     double *rootSlot = nullptr)
 {
     double xs[NC];
     for (int i = 0; i < NC; ++i) xs[i] = sval(x[i]);
     const double pV = sval(P), tV = sval(T);
     CpaMix<NC, double> sm;
     cpaMixing<NC, double>(c, xs, sm);
-    const double root = Coop ? cpaDensityRootValueCoop<NC>(f, c, pV, tV, xs, liquid, sm, err)
-                             : cpaDensityRootValue<NC>(f, c, pV, tV, xs, liquid, sm, err);
+    const bool cached = rootSlot && rootSlot[0] > 0.0;
+    double root;
+    if (cached)
+        root = rootSlot[0];
+    else
+    {
+        root = Coop ? cpaDensityRootValueCoop<NC>(f, c, pV, tV, xs, liquid, sm, err)
+                    : cpaDensityRootValue<NC>(f, c, pV, tV, xs, liquid, sm, err);
+    }
     S rho = P;
     if (err.bad())
         return rho;
@@ -1006,15 +1018,23 @@ __host__ __device__ S cpaDensityRoot(const FluidParams &f, const CpaTemp<NC> &c,
     cpaMixing<NC, S>(c, x, dm);
     for (int k = 0; k < 2; ++k)
     {
-        const double rv = sval(rho);
-        const double h = maxd(1.0e-4, ::fabs(rv) * 2.0e-6);
-        const double lower = maxd(1.0e-12, rv - h);
-        const double upper = rv + h;
-        const double pLower = cpaPressure<NC, double>(f, c, lower, tV, xs, sm, err);
-        const double pUpper = cpaPressure<NC, double>(f, c, upper, tV, xs, sm, err);
-        if (err.bad())
-            return rho;
-        const double derivative = (pUpper - pLower) / (upper - lower);
+        double derivative;
+        if (cached)
+            derivative = rootSlot[1 + k];
+        else
+        {
+            const double rv = sval(rho);
+            const double h = maxd(1.0e-4, ::fabs(rv) * 2.0e-6);
+            const double lower = maxd(1.0e-12, rv - h);
+            const double upper = rv + h;
+            const double pLower = cpaPressure<NC, double>(f, c, lower, tV, xs, sm, err);
+            const double pUpper = cpaPressure<NC, double>(f, c, upper, tV, xs, sm, err);
+            if (err.bad())
+                return rho;
+            derivative = (pUpper - pLower) / (upper - lower);
+            if (rootSlot)
+                rootSlot[1 + k] = derivative;
+        }
         if (!finite(derivative) || ::fabs(derivative) < 1.0e-12)
         {
             err.raise(kErrThermo);
@@ -1025,6 +1045,8 @@ __host__ __device__ S cpaDensityRoot(const FluidParams &f, const CpaTemp<NC> &c,
             return rho;
         rho -= residual / derivative;
     }
+    if (rootSlot && !cached)
+        rootSlot[0] = root;
     return rho;
 }
 
@@ -1083,9 +1105,10 @@ __host__ __device__ void cpaAtDensity(const FluidParams &f, const CpaTemp<NC> &c
 // ln(phi).  liquid = role is not gas.
 template <int NC, class S, bool Coop = false>
 __host__ __device__ void cpaPhase(const FluidParams &f, const CpaTemp<NC> &c, const S &P,
-                                  const S &T, const S *x, bool liquid, S &Z, S *lnPhi, Err &err)
+                                  const S &T, const S *x, bool liquid, S &Z, S *lnPhi, Err &err,
+                                  double *rootSlot = nullptr)
 {
-    const S rho = cpaDensityRoot<NC, S, Coop>(f, c, P, T, x, liquid, err);
+    const S rho = cpaDensityRoot<NC, S, Coop>(f, c, P, T, x, liquid, err, rootSlot);
     if (err.bad())
         return;
     cpaAtDensity<NC, S>(f, c, P, T, x, rho, Z, lnPhi, err);

@@ -25,6 +25,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "aqueous.hpp"
 #include "cpa.hpp"
 #include "eos.hpp"
 
@@ -379,9 +380,12 @@ public:
         t = r.rho[a]; r.rho[a] = r.rho[b]; r.rho[b] = t;
     }
 
-    // Aqueous composition-domain closures are not configured by the supported
-    // fluids, so every finite composition is admissible (cubic_eos.hpp:120-139).
-    __host__ __device__ bool aqueousSupported(const double *) const { return true; }
+    // The Water-role closures' composition domain (cubic_eos.hpp:449-468): every
+    // composition when none is configured.
+    __host__ __device__ bool aqueousSupported(const double *x) const
+    {
+        return aqueousDomain<NC>(f, x);
+    }
 
     __host__ __device__ bool waterIsDominant(const double *x) const
     {
@@ -416,7 +420,21 @@ public:
                 err.raise(kErrThermo);
                 return;
             }
-            const double rhoM = P / (kGasConstant * T * Z);
+            const bool waterRole = roleOf(p) == kWater;
+            if (waterRole && !aqueousSupported(r.x[p]))
+            {
+                // outside the Water-role closures' domain: not a converged state
+                r.converged = false;
+                for (int q = 0; q < 3; ++q)
+                {
+                    r.sat[q] = 0.0;
+                    r.rho[q] = 0.0;
+                }
+                return;
+            }
+            const double rhoM = phaseMolarDensity<NC, double>(f, P, T, r.x[p], Z, waterRole, err);
+            if (err.bad())
+                return;
             r.rho[p] = rhoM;
             volume[p] = maxd(r.beta[p], 0.0) / rhoM;
             volumeSum += volume[p];

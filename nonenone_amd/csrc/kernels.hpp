@@ -226,18 +226,22 @@ __global__ void __launch_bounds__(kBlock, kWavesPerEu) flashKernel(
     });
 }
 
-// The baseline form: one work-item per state in index order, no queue, no deferral.
+// The baseline form's first kernel: one work-item per state in index order (no
+// persistent queue, no sort); a state past the lane budget goes to the same
+// cooperative pass as in the scheduled form, so both forms compute every state
+// by the same numerics.
 template <int NC>
 __global__ void __launch_bounds__(kBlock, kWavesPerEu) flashPlainKernel(
     const FluidParams *__restrict__ fp, const FlashOptions *__restrict__ op, int64_t n,
     const double *__restrict__ P, const double *__restrict__ T, const double *__restrict__ z,
     const uint8_t *__restrict__ allowed, const double *__restrict__ seed, FlashHint hint,
-    nn_flash_out out)
+    nn_flash_out out, WorkQueue *__restrict__ q, unsigned int *__restrict__ list)
 {
     const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (s >= n)
         return;
-    flashOne<NC, kModePlain>(*fp, *op, s, P, T, z, allowed, seed, hint, out);
+    if (q->deferAll || !flashOne<NC, kModeDefer>(*fp, *op, s, P, T, z, allowed, seed, hint, out))
+        deferItem(q, list, s);
 }
 
 // Sort keys that put states of similar (T, P, z) next to each other, so that
@@ -538,17 +542,21 @@ __global__ void __launch_bounds__(256, kWavesPerEu) cellRecordKernel(
     Err err;
     CellStateT<NC, S> st;
     CellPropsT<NC, S> pr;
+    double roots[9] = {-1.0, 0.0, 0.0, -1.0, 0.0, 0.0, -1.0, 0.0, 0.0};
     if (!loadCell<NC, Well>(f, v, primary, phaseState, c, presence))
         err.raise(kErrThermo);
     else
     {
         decodeState<NC, Well, S>(primary + c * L::NP, presence, st);
-        evaluateProperties<NC, S>(f, st, v.porosity[c], v.T[c], pr, err);
+        evaluateProperties<NC, S>(f, st, v.porosity[c], v.T[c], pr, err,
+                                  v.roots ? roots : nullptr);
     }
     if (status)
         status[c] = err.code;
     const double nan = __builtin_nan("");
     const int64_t n = v.ncell;
+    if (v.roots)
+        for (int k = 0; k < 9; ++k) v.roots[k * n + c] = err.bad() ? -1.0 : roots[k];
     if (v.record)
     {
         double *r = v.record;
@@ -568,7 +576,15 @@ __global__ void __launch_bounds__(256, kWavesPerEu) cellRecordKernel(
             accum[c * NC + k] = err.bad() ? nan : componentAccumulation<NC, S>(pr, k).v;
 }
 
-// Residual + Jacobian column of one cell (residual.inc:59-120, jacobian.inc:20-237).
+// Residual + Jacobian columns of one cell (residual.inc:59-120, jacobian.inc:20-237),
+// in derivative chunks: chunk k evaluates the cell with Dual<CH> seeded on primary
+// variables [k CH, k CH + CH).  A derivative component is computed by the same
+// operations on the same values whatever the other components hold, so every
+// column is bit-identical to a Dual<N_p> evaluation, while a lane keeps CH + 1
+// doubles per AD quantity instead of N_p + 1 (private segment and spills shrink
+// with it).  The first chunk also computes the residual row and the upwind
+// decisions; the PR-CPA density roots with their correction slopes come from the
+// record pass (or, without it, from the first chunk) and are reused by all chunks.
 template <int NC, bool Well>
 __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
     const FluidParams *__restrict__ fp, AssemblyView v, double dt, double fugScale,
@@ -578,136 +594,180 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
 {
     using L = Layout<NC, Well>;
     constexpr int NP = L::NP;
-    using S = Dual<NP>;
+    constexpr int CH = kAssemblyChunk;
+    constexpr int NCHUNK = (NP + CH - 1) / CH;
+    using S = Dual<CH>;
     const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= v.ncell)
         return;
     const FluidParams &f = *fp;
-    Err err;
     uint8_t presence = 0;
-    CellStateT<NC, S> st;
-    CellPropsT<NC, S> pr;
-    if (!loadCell<NC, Well>(f, v, primary, phaseState, c, presence))
-        err.raise(kErrThermo);
-    else
-    {
-        decodeState<NC, Well, S>(primary + c * NP, presence, st);
-        evaluateProperties<NC, S>(f, st, v.porosity[c], v.T[c], pr, err);
-    }
-    const double nan = __builtin_nan("");
-    if (err.bad())
-    {
-        if (status) status[c] = err.code;
-        if (residual)
-            for (int r = 0; r < NP; ++r) residual[c * NP + r] = nan;
-        if (bsr)
-        {
-            double *self = bsr + v.selfSlot[c] * NP * NP;
-            for (int e = 0; e < NP * NP; ++e) self[e] = nan;
-            for (int64_t k = v.connOffsets[c]; k < v.connOffsets[c + 1]; ++k)
-            {
-                double *rb = bsr + v.recipSlot[k] * NP * NP;
-                for (int e = 0; e < NP * NP; ++e) rb[e] = nan;
-            }
-        }
-        return;
-    }
-    S res[NP];
-    localResidual<NC, Well, S>(st, pr, accumPrev + c * NC, v.volume[c], dt, fugScale, res);
-
+    const bool loaded = loadCell<NC, Well>(f, v, primary, phaseState, c, presence);
     // row scale of each equation of this cell (natural_petsc_runtime_scaling.inc:26-89)
     double rowScale[NP];
     for (int r = 0; r < NP; ++r) rowScale[r] = v.rowScale[r];
     if constexpr (Well)
         if (v.wellRowScale)
             rowScale[NP - 1] = v.wellRowScale[c];
-    // jacobian.inc:262-268: (derivative * rowScale) * variableScale
     double *self = bsr ? bsr + v.selfSlot[c] * NP * NP : nullptr;
-    if (self)
-        for (int r = 0; r < NP; ++r)
-            for (int q = 0; q < NP; ++q) self[r * NP + q] = res[r].d[q] * rowScale[r] * v.varScale[q];
-
-    double outward[NC];
-    for (int k = 0; k < NC; ++k) outward[k] = 0.0;
     const int64_t n = v.ncell;
     const double *rec = v.record;
     const double vIn = v.volume[c];
-    for (int64_t k = v.connOffsets[c]; k < v.connOffsets[c + 1]; ++k)
+    const int64_t kBegin = v.connOffsets[c], kEnd = v.connOffsets[c + 1];
+    // density roots and correction slopes of the record pass (PR-CPA), which
+    // evaluated the same cell values
+    double roots[9] = {-1.0, 0.0, 0.0, -1.0, 0.0, 0.0, -1.0, 0.0, 0.0};
+    if (v.roots)
+        for (int k = 0; k < 9; ++k) roots[k] = v.roots[k * v.ncell + c];
+    for (int chunk = 0; chunk < NCHUNK; ++chunk)
     {
-        const int64_t nb = v.connNeighbor[k];
-        const double trans = v.connTrans[k];
-        const double gdz = v.connGdz[k];
-        const double pEx = rec[nb];
-        const double vEx = v.volume[nb];
-        bool upIn[3];
-        double faceFlux[NC];
-        for (int i = 0; i < NC; ++i) faceFlux[i] = 0.0;
-        // scalar face flux (face_flux.hpp:330-430)
-        for (int q = 0; q < 3; ++q)
+        const int col0 = chunk * CH;
+        Err err;
+        CellStateT<NC, S> st;
+        CellPropsT<NC, S> pr;
+        if (!loaded)
+            err.raise(kErrThermo);
+        else
         {
-            const double rhoIn = pr.rho[q].v, rhoEx = rec[(1 + q) * n + nb];
-            const double satIn = pr.sat[q].v, satEx = rec[(4 + q) * n + nb];
-            const double avg = 0.5 * (rhoIn * satIn + rhoEx * satEx) /
-                               maxd(0.5 * (satIn + satEx), 1.0e-8);
-            const double dPhi = pEx + avg * gdz - st.p.v;
-            upIn[q] = interiorUpwind(dPhi, vIn, vEx, c, nb);
-            if (decisions)
-                decisions[k * 3 + q] = upIn[q] ? 0 : 1;
-            const double mob = upIn[q] ? pr.mob[q].v : rec[(7 + q) * n + nb];
-            const double dens = upIn[q] ? rhoIn : rhoEx;
-            const double total = -trans * dPhi * mob * dens;
-            for (int i = 0; i < NC; ++i)
+            decodeState<NC, Well, S>(primary + c * NP, presence, st, col0);
+            evaluateProperties<NC, S>(f, st, v.porosity[c], v.T[c], pr, err, roots);
+        }
+        if (err.bad())
+        {
+            // values are those of the first chunk, so only the first chunk gets here
+            const double nan = __builtin_nan("");
+            if (status) status[c] = err.code;
+            if (residual)
+                for (int r = 0; r < NP; ++r) residual[c * NP + r] = nan;
+            if (bsr)
             {
-                const double wv = upIn[q] ? pr.w[q][i].v : rec[(10 + q * NC + i) * n + nb];
-                faceFlux[i] += total * wv;
+                for (int e = 0; e < NP * NP; ++e) self[e] = nan;
+                for (int64_t k = kBegin; k < kEnd; ++k)
+                {
+                    double *rb = bsr + v.recipSlot[k] * NP * NP;
+                    for (int e = 0; e < NP * NP; ++e) rb[e] = nan;
+                }
+            }
+            return;
+        }
+        S res[NP];
+        localResidual<NC, Well, S>(st, pr, accumPrev + c * NC, v.volume[c], dt, fugScale, res);
+
+        // jacobian.inc:262-268: (derivative * rowScale) * variableScale; the mass rows
+        // gather the face-flux derivatives below in the order of the connections
+        double selfMass[NC][CH];
+        if (self)
+        {
+            for (int q = 0; q < CH; ++q)
+            {
+                const int col = col0 + q;
+                if (col >= NP)
+                    break;
+                for (int r = 0; r < NC; ++r)
+                    selfMass[r][q] = res[r].d[q] * rowScale[r] * v.varScale[col];
+                for (int r = NC; r < NP; ++r)
+                    self[r * NP + col] = res[r].d[q] * rowScale[r] * v.varScale[col];
             }
         }
-        for (int i = 0; i < NC; ++i) outward[i] += faceFlux[i];
-        if (!self)
-            continue;
-        // AD face flux with the recorded decision (face_flux.hpp:174-321)
-        S flux[NC];
-        for (int i = 0; i < NC; ++i) flux[i] = S(0.0);
-        for (int q = 0; q < 3; ++q)
+        double outward[NC];
+        for (int k = 0; k < NC; ++k) outward[k] = 0.0;
+        for (int64_t k = kBegin; k < kEnd; ++k)
         {
-            const double rhoEx = rec[(1 + q) * n + nb];
-            const double satEx = rec[(4 + q) * n + nb];
-            const S satAvg = 0.5 * (pr.sat[q] + satEx);
-            const S avg = 0.5 * (pr.rho[q] * pr.sat[q] + rhoEx * satEx) / maxd(satAvg.v, 1.0e-8);
-            const S dPhi = pEx + avg * gdz - st.p;
-            if (upIn[q])
+            const int64_t nb = v.connNeighbor[k];
+            const double trans = v.connTrans[k];
+            const double gdz = v.connGdz[k];
+            const double pEx = rec[nb];
+            const double vEx = v.volume[nb];
+            bool upIn[3];
+            double faceFlux[NC];
+            for (int i = 0; i < NC; ++i) faceFlux[i] = 0.0;
+            // scalar face flux (face_flux.hpp:330-430): the upwind decisions, and in
+            // the first chunk the residual's flux
+            for (int q = 0; q < 3; ++q)
             {
-                const S darcy = -trans * dPhi * pr.mob[q];
-                const S total = darcy * pr.rho[q];
-                for (int i = 0; i < NC; ++i) flux[i] += total * pr.w[q][i];
-            }
-            else
-            {
-                const double mobEx = rec[(7 + q) * n + nb];
-                const S darcy = -trans * dPhi * mobEx;
-                const S total = darcy * rhoEx;
+                const double rhoIn = pr.rho[q].v, rhoEx = rec[(1 + q) * n + nb];
+                const double satIn = pr.sat[q].v, satEx = rec[(4 + q) * n + nb];
+                const double avg = 0.5 * (rhoIn * satIn + rhoEx * satEx) /
+                                   maxd(0.5 * (satIn + satEx), 1.0e-8);
+                const double dPhi = pEx + avg * gdz - st.p.v;
+                upIn[q] = interiorUpwind(dPhi, vIn, vEx, c, nb);
+                if (chunk != 0)
+                    continue;
+                if (decisions)
+                    decisions[k * 3 + q] = upIn[q] ? 0 : 1;
+                const double mob = upIn[q] ? pr.mob[q].v : rec[(7 + q) * n + nb];
+                const double dens = upIn[q] ? rhoIn : rhoEx;
+                const double total = -trans * dPhi * mob * dens;
                 for (int i = 0; i < NC; ++i)
-                    flux[i] += total * rec[(10 + q * NC + i) * n + nb];
+                {
+                    const double wv = upIn[q] ? pr.w[q][i].v : rec[(10 + q * NC + i) * n + nb];
+                    faceFlux[i] += total * wv;
+                }
             }
-        }
-        // jacobian.inc:196-213: mass rows carry one scale in every cell
-        double *rb = bsr + v.recipSlot[k] * NP * NP;
-        for (int i = 0; i < NC; ++i)
-            for (int q = 0; q < NP; ++q)
+            if (chunk == 0)
+                for (int i = 0; i < NC; ++i) outward[i] += faceFlux[i];
+            if (!self)
+                continue;
+            // AD face flux with the recorded decision (face_flux.hpp:174-321)
+            S flux[NC];
+            for (int i = 0; i < NC; ++i) flux[i] = S(0.0);
+            for (int q = 0; q < 3; ++q)
             {
-                self[i * NP + q] += flux[i].d[q] * rowScale[i] * v.varScale[q];
-                rb[i * NP + q] = -flux[i].d[q] * v.rowScale[i] * v.varScale[q];
+                const double rhoEx = rec[(1 + q) * n + nb];
+                const double satEx = rec[(4 + q) * n + nb];
+                const S satAvg = 0.5 * (pr.sat[q] + satEx);
+                const S avg = 0.5 * (pr.rho[q] * pr.sat[q] + rhoEx * satEx) / maxd(satAvg.v, 1.0e-8);
+                const S dPhi = pEx + avg * gdz - st.p;
+                if (upIn[q])
+                {
+                    const S darcy = -trans * dPhi * pr.mob[q];
+                    const S total = darcy * pr.rho[q];
+                    for (int i = 0; i < NC; ++i) flux[i] += total * pr.w[q][i];
+                }
+                else
+                {
+                    const double mobEx = rec[(7 + q) * n + nb];
+                    const S darcy = -trans * dPhi * mobEx;
+                    const S total = darcy * rhoEx;
+                    for (int i = 0; i < NC; ++i)
+                        flux[i] += total * rec[(10 + q * NC + i) * n + nb];
+                }
             }
-        for (int r = NC; r < NP; ++r)
-            for (int q = 0; q < NP; ++q) rb[r * NP + q] = 0.0;
-    }
-    if (residual)
-    {
-        // natural_petsc_runtime_common.inc:345-356
-        for (int i = 0; i < NC; ++i)
-            residual[c * NP + i] = (res[i].v + outward[i]) * rowScale[i];
-        for (int r = NC; r < NP; ++r)
-            residual[c * NP + r] = res[r].v * rowScale[r];
+            // jacobian.inc:196-213: mass rows carry one scale in every cell
+            double *rb = bsr + v.recipSlot[k] * NP * NP;
+            for (int q = 0; q < CH; ++q)
+            {
+                const int col = col0 + q;
+                if (col >= NP)
+                    break;
+                for (int i = 0; i < NC; ++i)
+                {
+                    selfMass[i][q] += flux[i].d[q] * rowScale[i] * v.varScale[col];
+                    rb[i * NP + col] = -flux[i].d[q] * v.rowScale[i] * v.varScale[col];
+                }
+            }
+            if (chunk == 0)
+                for (int r = NC; r < NP; ++r)
+                    for (int q = 0; q < NP; ++q) rb[r * NP + q] = 0.0;
+        }
+        if (self)
+            for (int q = 0; q < CH; ++q)
+            {
+                const int col = col0 + q;
+                if (col >= NP)
+                    break;
+                for (int r = 0; r < NC; ++r) self[r * NP + col] = selfMass[r][q];
+            }
+        if (chunk == 0 && residual)
+        {
+            // natural_petsc_runtime_common.inc:345-356
+            for (int i = 0; i < NC; ++i)
+                residual[c * NP + i] = (res[i].v + outward[i]) * rowScale[i];
+            for (int r = NC; r < NP; ++r)
+                residual[c * NP + r] = res[r].v * rowScale[r];
+        }
+        if (!self)
+            break;
     }
     if (status)
         status[c] = NN_STATUS_OK;
@@ -733,13 +793,11 @@ hipError_t Launch<NC>::flash(unsigned grid, unsigned coopGrid, hipStream_t st,
                              hipEvent_t splitB)
 {
     if (plain)
-    {
         flashPlainKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, hint,
-                                                      out);
-        return hipGetLastError();
-    }
-    flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, hint, out, q,
-                                             list, order);
+                                                      out, q, list);
+    else
+        flashKernel<NC><<<grid, kBlock, 0, st>>>(fp, op, n, P, T, z, allowed, seed, hint, out, q,
+                                                 list, order);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;

@@ -32,12 +32,16 @@ constexpr int kBlock = 64;  // one wavefront per block: divergent per-state cont
 #endif
 constexpr int kWavesPerEu = NN_WAVES_PER_EU;
 
-// Largest component count the Natural assembly is built for.  Its forward-mode
-// AD carries (3 NC + 1)-wide dual numbers per lane; at eight components the
-// assembly kernel's private segment is about 85 KB per lane, and its dispatch
-// stopped the queue with HSA_STATUS_ERROR_OUT_OF_RESOURCES (scratch), so larger
-// fluids get flash and phase updates but no assembly.
-constexpr int kAssemblyMaxNc = 6;
+// Derivative columns per chunk of the assembly kernel's forward-mode AD: a lane
+// evaluates its cell once per chunk of primary variables with Dual<kAssemblyChunk>.
+// Build parameter (nonenone_amd/build.py, NN_ASSEMBLY_CHUNK).
+#ifndef NN_ASSEMBLY_CHUNK
+#define NN_ASSEMBLY_CHUNK 2
+#endif
+constexpr int kAssemblyChunk = NN_ASSEMBLY_CHUNK;
+
+// Largest component count the Natural assembly is built for.
+constexpr int kAssemblyMaxNc = NN_MAX_NC;
 
 constexpr int kMaxNp = 3 * NN_MAX_NC + 2;  // primary variables with a well unknown
 
@@ -54,6 +58,10 @@ struct AssemblyView
     const int64_t *selfSlot;   // BSR slot of block (c, c)
     const int64_t *recipSlot;  // per connection: BSR slot of block (neighbour, c)
     double *record;            // neighbour property records, field-major (SoA)
+    // PR-CPA (else null): per cell and phase, the density root and its two
+    // correction slopes from the record pass, field-major (9 fields), which the
+    // assembly pass reuses instead of repeating the root search.
+    double *roots;
     // Equation/variable scaling of the Natural runtime (all 1 when disabled):
     // residual[r] *= rowScale[r]; J[r][q] = (dR_r/dx_q * rowScale[r]) * varScale[q].
     // wellRowScale (per cell, may be null) replaces rowScale of the well-control row.
@@ -105,8 +113,8 @@ struct Launch
                           double *Z, double *lnphi, double *fug, int32_t *status);
     // Lane-per-state pass, then one wavefront per deferred state.  `splitA`
     // and `splitB` (may be null) are recorded between the two launches.
-    // `plain`: one kernel, one work-item per state in index order, no deferral
-    // (the baseline form; `grid` is then ceil(n / kBlock)).
+    // `plain`: one work-item per state in index order instead of the persistent
+    // queue (the baseline form; `grid` is then ceil(n / kBlock)).
     static hipError_t flash(unsigned grid, unsigned coopGrid, hipStream_t st,
                             const FluidParams *fp, const FlashOptions *op, int64_t n,
                             const double *P, const double *T, const double *z,

@@ -21,6 +21,7 @@
 #include "common.hpp"
 #include "cpa.hpp"
 #include "launch.hpp"
+#include "newton_limiter.hpp"
 
 using namespace nn;
 
@@ -104,6 +105,19 @@ int dispatchNc(int nc, F &&f)
 
 // Work-queue counters reset by a kernel rather than hipMemsetAsync, so that
 // the launch sequence stays valid inside a captured HIP graph.
+template <int NC, bool Well>
+__global__ void newtonLimitKernel(int64_t n, const double *__restrict__ primary,
+                                  double *__restrict__ step, AssemblyView v, double maxAqueous,
+                                  int water, int co2)
+{
+    using L = Layout<NC, Well>;
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n)
+        return;
+    limitNewtonCell<NC, Well>(primary + c * L::NP, step + c * L::NP, v.varScale, maxAqueous,
+                              water, co2);
+}
+
 __global__ void resetQueueKernel(WorkQueue *q, unsigned int deferAll, unsigned int claim)
 {
     q->head = 0u;
@@ -422,6 +436,12 @@ FluidParams toParams(const nn_fluid_desc &d)
     p.cpaMaxIter = d.cpa_max_iterations;
     p.cpaTol = d.cpa_tolerance;
     p.cpaDamping = d.cpa_damping;
+    p.aqVolume = d.aqueous_volume;
+    p.aqViscosity = d.aqueous_viscosity;
+    p.co2 = d.co2_component;
+    p.waterMolarMass = d.water_molar_mass;
+    p.aqVolumeMax = d.aqueous_volume_max_unsupported;
+    p.aqViscosityMax = d.aqueous_viscosity_max_unsupported;
     return p;
 }
 
@@ -571,6 +591,28 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
     if (fluid->thermo_model == NN_MODEL_SW &&
         (fluid->water_component < 0 || fluid->water_component >= fluid->nc))
         return fail(NN_E_INVALID_ARGUMENT, "SW water-component index is out of range");
+    // Water-role closures (IapwsGarciaAqueousVolume / McBrideWright2015AqueousViscosity /
+    // Iapws2008IndustrialAqueousViscosity constructors: aqueous_volume.hpp:347-361,
+    // aqueous_viscosity.hpp:33-41, 114-128)
+    if (fluid->aqueous_volume != 0 && fluid->aqueous_volume != 1)
+        return fail(NN_E_INVALID_ARGUMENT, "aqueous_volume must be 0 or 1");
+    if (fluid->aqueous_viscosity < NN_AQ_VISCOSITY_NONE ||
+        fluid->aqueous_viscosity > NN_AQ_VISCOSITY_IAPWS2008)
+        return fail(NN_E_INVALID_ARGUMENT, "unknown aqueous_viscosity model");
+    const bool needsCo2 = fluid->aqueous_volume != 0 ||
+                          fluid->aqueous_viscosity == NN_AQ_VISCOSITY_MCBRIDE_WRIGHT;
+    if ((needsCo2 || fluid->aqueous_viscosity != NN_AQ_VISCOSITY_NONE) &&
+        (fluid->water_component < 0 || fluid->water_component >= fluid->nc))
+        return fail(NN_E_INVALID_ARGUMENT, "aqueous closures need a water component");
+    if (needsCo2 && (fluid->co2_component < 0 || fluid->co2_component >= fluid->nc ||
+                     fluid->co2_component == fluid->water_component))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid aqueous H2O/CO2 component indices");
+    if (fluid->aqueous_volume != 0 &&
+        (!(fluid->water_molar_mass > 0.0) || !(fluid->aqueous_volume_max_unsupported >= 0.0)))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid IAPWS-Garcia configuration");
+    if (fluid->aqueous_viscosity != NN_AQ_VISCOSITY_NONE &&
+        !(fluid->aqueous_viscosity_max_unsupported >= 0.0))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid aqueous-viscosity configuration");
     for (int i = 0; i < fluid->nc; ++i)
         for (int j = 0; j < fluid->nc; ++j)
             if (fluid->kij[i * NN_MAX_NC + j] != fluid->kij[j * NN_MAX_NC + i])
@@ -646,6 +688,20 @@ int nn_context_kernel_time(nn_context *ctx, int32_t kind, double *total_ms, int6
     return NN_OK;
 }
 
+int nn_context_queue_counts(nn_context *ctx, int64_t *deferred, int64_t *recorded,
+                            int64_t *reflashed)
+{
+    if (!ctx || !deferred || !recorded || !reflashed)
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    NN_HIP(hipDeviceSynchronize());
+    WorkQueue q{};
+    NN_HIP(hipMemcpy(&q, ctx->dQueue, sizeof q, hipMemcpyDeviceToHost));
+    *deferred = q.deferred;
+    *recorded = q.pending;
+    *reflashed = q.reflash;
+    return NN_OK;
+}
+
 int nn_context_destroy(nn_context *ctx)
 {
     if (!ctx)
@@ -700,24 +756,11 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
         return fail(NN_E_INVALID_ARGUMENT, "batch too large for the 32-bit work queue");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const nn_flash_out o = *out;
-    if (ctx->sched.plain)
-    {
-        // Baseline form: one kernel, one work-item per state, index order.
-        return dispatchNc(ctx->host.nc, [&](auto ncTag) {
-            constexpr int NC = decltype(ncTag)::value;
-            const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-            LaunchTimer t(ctx, NN_KERNEL_FLASH, st);
-            NN_HIP((Launch<NC>::flash(grid, 0, st, ctx->dFluid, ctx->dOpts, n, d_P, d_T, d_z,
-                                      d_allowed, d_seed, hint, o, nullptr, nullptr, nullptr, true,
-                                      nullptr, nullptr)));
-            t.done();
-            return NN_OK;
-        });
-    }
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
-    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems && n <= kOrderMaxItems &&
-                         !streamCapturing(st);
+    const bool plain = ctx->sched.plain != 0;
+    const bool ordered = !plain && ctx->sched.ordered && n >= kOrderMinItems &&
+                         n <= kOrderMaxItems && !streamCapturing(st);
     if (ordered)
         if (const int rc = ensureSort(ctx, n); rc != NN_OK)
             return rc;
@@ -735,9 +778,12 @@ static int launchFlash(nn_context *ctx, int64_t n, const double *d_P, const doub
         }
         LaunchTimer t(ctx, NN_KERNEL_FLASH, st);
         t.makeSplit();
-        NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), coopGrid(ctx), st, ctx->dFluid,
-                                  ctx->dOpts, n, d_P, d_T, d_z, d_allowed, d_seed, hint, o,
-                                  ctx->dQueue, ctx->dList, order, false, t.splitA, t.splitB)));
+        // the baseline form: one work-item per state (ceil(n / 64) blocks), index order
+        const unsigned grid = plain ? static_cast<unsigned>((n + kBlock - 1) / kBlock)
+                                    : persistentGrid(ctx, n);
+        NN_HIP((Launch<NC>::flash(grid, coopGrid(ctx), st, ctx->dFluid, ctx->dOpts, n, d_P, d_T,
+                                  d_z, d_allowed, d_seed, hint, o, ctx->dQueue, ctx->dList, order,
+                                  plain, t.splitA, t.splitB)));
         t.doneSplit(NN_KERNEL_FLASH_COOP);
         return NN_OK;
     });
@@ -785,21 +831,16 @@ int nn_flash_batch_static(nn_context *ctx, int64_t n, const double *d_P, const d
     const FlashHint none{nullptr, nullptr};
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
-        if (ctx->sched.plain)
-        {
-            const unsigned grid = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-            NN_HIP((Launch<NC>::flash(grid, 0, st, ctx->dFluid, ctx->dOpts, n, d_P, d_T, d_z,
-                                      nullptr, nullptr, none, o, nullptr, nullptr, nullptr, true,
-                                      nullptr, nullptr)));
-            return NN_OK;
-        }
         // the queue's counters start at zero (a kernel, not a memset: capturable)
         resetQueueKernel<<<1, 1, 0, st>>>(q, ctx->coopAll ? 1u : 0u,
                                           ctx->sched.wave_claim ? 1u : 0u);
         NN_HIP(hipGetLastError());
-        NN_HIP((Launch<NC>::flash(persistentGrid(ctx, n), coopGrid(ctx), st, ctx->dFluid,
-                                  ctx->dOpts, n, d_P, d_T, d_z, nullptr, nullptr, none, o, q,
-                                  list, nullptr, false, nullptr, nullptr)));
+        const bool plain = ctx->sched.plain != 0;
+        const unsigned grid = plain ? static_cast<unsigned>((n + kBlock - 1) / kBlock)
+                                    : persistentGrid(ctx, n);
+        NN_HIP((Launch<NC>::flash(grid, coopGrid(ctx), st, ctx->dFluid, ctx->dOpts, n, d_P, d_T,
+                                  d_z, nullptr, nullptr, none, o, q, list, nullptr, plain,
+                                  nullptr, nullptr)));
         return NN_OK;
     });
 }
@@ -819,11 +860,6 @@ int nn_context_set_schedule(nn_context *ctx, const nn_schedule *s)
     if (s->persistent_blocks_per_cu < 1 || s->persistent_blocks_per_cu > 64 ||
         s->coop_blocks_per_cu < 1 || s->coop_blocks_per_cu > 64)
         return fail(NN_E_INVALID_ARGUMENT, "blocks per CU must be in 1..64");
-    // The plain form runs every recovery path in one lane; its six-component
-    // build returned results that differ from the two-pass form's (which
-    // matches the reference), so it is offered for three components only.
-    if (s->plain && ctx->host.nc != 3)
-        return fail(NN_E_UNSUPPORTED, "the plain one-kernel form is built for three components");
     ctx->sched = *s;
     ctx->sched.ordered = s->ordered ? 1 : 0;
     ctx->sched.wave_claim = s->wave_claim ? 1 : 0;
@@ -1182,6 +1218,7 @@ struct nn_assembly
     int64_t *dSelf = nullptr;
     int64_t *dRecip = nullptr;
     double *dRecord = nullptr;
+    double *dRoots = nullptr;
 };
 
 extern "C" {
@@ -1205,6 +1242,7 @@ int nn_assembly_destroy(nn_assembly *a)
     if (a->dSelf) (void)hipFree(a->dSelf);
     if (a->dRecip) (void)hipFree(a->dRecip);
     if (a->dRecord) (void)hipFree(a->dRecord);
+    if (a->dRoots) (void)hipFree(a->dRoots);
     delete a;
     return NN_OK;
 }
@@ -1223,7 +1261,7 @@ int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
     if (ctx->host.water < 0)
         return fail(NN_E_INVALID_ARGUMENT, "Natural assembly needs a water component");
     if (ctx->host.nc > kAssemblyMaxNc)
-        return fail(NN_E_UNSUPPORTED, "Natural assembly is built for at most 6 components");
+        return fail(NN_E_UNSUPPORTED, "Natural assembly is built for at most NN_MAX_NC components");
     // Slots of the diagonal and reciprocal blocks in the cached pattern (setup only).
     std::vector<int64_t> co(ncell + 1), ro(ncell + 1);
     NN_HIP(hipMemcpy(co.data(), d_conn_offsets, sizeof(int64_t) * (ncell + 1), hipMemcpyDeviceToHost));
@@ -1268,9 +1306,11 @@ int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
     a->nnzb = nnzb;
     const int nc = ctx->host.nc;
     const size_t recordBytes = sizeof(double) * size_t(10 + 3 * nc) * size_t(ncell);
+    const bool cpa = ctx->host.model == NN_MODEL_CPA;
     if (hipMalloc(&a->dSelf, sizeof(int64_t) * ncell) != hipSuccess ||
         hipMalloc(&a->dRecip, sizeof(int64_t) * std::max<int64_t>(1, nconn)) != hipSuccess ||
-        hipMalloc(&a->dRecord, recordBytes) != hipSuccess)
+        hipMalloc(&a->dRecord, recordBytes) != hipSuccess ||
+        (cpa && hipMalloc(&a->dRoots, sizeof(double) * 9 * size_t(ncell)) != hipSuccess))
     {
         nn_assembly_destroy(a);
         return fail(NN_E_OUT_OF_MEMORY, "device allocation failed");
@@ -1280,7 +1320,7 @@ int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
         NN_HIP(hipMemcpy(a->dRecip, recip.data(), sizeof(int64_t) * nconn, hipMemcpyHostToDevice));
     a->view = AssemblyView{ncell, d_conn_offsets, d_conn_neighbor, d_conn_trans, d_conn_gdz,
                            d_volume, d_porosity, d_T, a->dSelf, a->dRecip, a->dRecord,
-                           nullptr, {}, {}};
+                           a->dRoots, nullptr, {}, {}};
     for (int k = 0; k < kMaxNp; ++k)
         a->view.rowScale[k] = a->view.varScale[k] = 1.0;
     *out = a;
@@ -1347,6 +1387,38 @@ int nn_assembly_set_scaling(nn_assembly *a, const nn_scaling *s, const double *d
     return NN_OK;
 }
 
+int nn_newton_limit(nn_assembly *a, const double *d_primary, double *d_step,
+                    double max_aqueous_unsupported, int32_t water_component,
+                    int32_t co2_component, void *stream)
+{
+    if (!a || !d_primary || !d_step)
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    const int nc = a->ctx->host.nc;
+    // natural_petsc_runtime_common.inc:43-60
+    if (max_aqueous_unsupported >= 0.0 &&
+        (!finite(max_aqueous_unsupported) || water_component < 0 || water_component >= nc ||
+         co2_component < 0 || co2_component >= nc || water_component == co2_component))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid aqueous Newton composition-domain configuration");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t n = a->view.ncell;
+    if (n == 0)
+        return NN_OK;
+    const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+    return dispatchNc(nc, [&](auto ncTag) {
+        constexpr int NC = decltype(ncTag)::value;
+        if (a->hasWell)
+            newtonLimitKernel<NC, true><<<grid, 256, 0, st>>>(n, d_primary, d_step, a->view,
+                                                              max_aqueous_unsupported,
+                                                              water_component, co2_component);
+        else
+            newtonLimitKernel<NC, false><<<grid, 256, 0, st>>>(n, d_primary, d_step, a->view,
+                                                               max_aqueous_unsupported,
+                                                               water_component, co2_component);
+        NN_HIP(hipGetLastError());
+        return NN_OK;
+    });
+}
+
 int nn_assembly_accumulation(nn_assembly *a, const double *d_primary, const double *d_phase_state,
                              double *d_accum, void *stream)
 {
@@ -1355,6 +1427,7 @@ int nn_assembly_accumulation(nn_assembly *a, const double *d_primary, const doub
     hipStream_t st = static_cast<hipStream_t>(stream);
     AssemblyView v = a->view;
     v.record = nullptr;
+    v.roots = nullptr;
     return dispatchNc(a->ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
         NN_HIP((Launch<NC>::record(a->hasWell != 0, st, a->ctx->dFluid, v, d_primary, d_phase_state,

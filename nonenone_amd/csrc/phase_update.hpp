@@ -296,7 +296,9 @@ struct PhaseUpdater
             if (err.bad())
                 return;
             ps.Zf[p] = Z;
-            const double rhoM = P / (kGasConstant * T * Z);
+            const double rhoM = phaseMolarDensity<NC, double>(f, P, T, comp[p], Z, p == kWater, err);
+            if (err.bad())
+                return;
             moles[p] = rhoM * maxd(sat[p], 0.0);
             total += moles[p];
         }

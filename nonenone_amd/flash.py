@@ -121,6 +121,14 @@ class Context:
                    "nn_context_set_schedule")
         return self.schedule()
 
+    def queue_counts(self) -> dict:
+        """Work counters of the last flash / phase-update launch: items handed to the
+        wavefront-per-item pass, recorded for the lockstep trials, held for re-flash."""
+        v = [ctypes.c_int64() for _ in range(3)]
+        _abi.check(self.lib.nn_context_queue_counts(self.handle, *(ctypes.byref(x) for x in v)),
+                   "nn_context_queue_counts")
+        return {"deferred": v[0].value, "recorded": v[1].value, "reflashed": v[2].value}
+
     def kernel_time(self, kind: str) -> tuple[float, int]:
         """(summed milliseconds, launches) of one kernel kind since timing was enabled."""
         ms = ctypes.c_double()

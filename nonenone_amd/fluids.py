@@ -87,6 +87,25 @@ def make_desc(names, kij, model: str = "PR", water: str = "H2O",
     return d
 
 
+def with_aqueous_closures(d: FluidDesc, viscosity: str = "mcbride_wright",
+                          volume: bool = True, co2: int = 1) -> FluidDesc:
+    """Switch on the Water-role closures of the reference's benchmark case
+    (case/h2o_co2_nc10_2d_benchmark/case_config.hpp:70-74): IAPWS-IF97 + Garcia
+    molar volume and McBride-Wright (or IAPWS-2008, `viscosity="iapws2008"`)
+    viscosity, water molar mass 0.018015268 kg/mol, unsupported-solute limits
+    1e-4.  `co2`: index of CO2 (1 in the presets)."""
+    from ._abi import NN_AQ_VISCOSITY_IAPWS2008, NN_AQ_VISCOSITY_MCBRIDE_WRIGHT, NN_AQ_VISCOSITY_NONE
+    d.aqueous_volume = 1 if volume else 0
+    d.aqueous_viscosity = {"mcbride_wright": NN_AQ_VISCOSITY_MCBRIDE_WRIGHT,
+                           "iapws2008": NN_AQ_VISCOSITY_IAPWS2008,
+                           "none": NN_AQ_VISCOSITY_NONE}[viscosity]
+    d.co2_component = co2
+    d.water_molar_mass = 0.018015268
+    d.aqueous_volume_max_unsupported = 1.0e-4
+    d.aqueous_viscosity_max_unsupported = 1.0e-4
+    return d
+
+
 def h2o_co2_nc10(model: str = "PR", bips: str = "workload") -> FluidDesc:
     """Three-component H2O/CO2/nC10 fluid (PR78, eos type 5)."""
     names = ["H2O", "CO2", "nC10"]

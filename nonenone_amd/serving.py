@@ -158,7 +158,9 @@ MEMBER_FLUIDS = {
     "nc5-SW": lambda: fluids.five_component("SW"),
     "nc6-PR": lambda: fluids.six_component("PR"),
     "nc6-SW": lambda: fluids.six_component("SW"),
-}   # the Natural assembly is built for up to six components
+    "nc8-PR": lambda: fluids.eight_component("PR"),
+    "nc8-SW": lambda: fluids.eight_component("SW"),
+}
 
 
 class Member:

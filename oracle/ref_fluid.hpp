@@ -87,6 +87,20 @@ MPMC::FluidSystem<Indices> makeFluid(const nn_fluid_desc &d, double temperature)
             cpa.associationDamping = d.cpa_damping;
         eos.configureCubicPlusAssociation(std::move(cpa));
     }
+    // Water-role closures, as case/include/case/fluid_factory.hpp:278-296, 447-461
+    // configures them
+    if (d.aqueous_volume)
+    {
+        typename MPMC::CubicEquationOfState<Indices>::AqueousVolumeOptions aqueous;
+        aqueous.waterComponent = d.water_component;
+        aqueous.co2Component = d.co2_component;
+        aqueous.waterMolarMass = d.water_molar_mass;
+        aqueous.maximumUnsupportedMoleFraction = d.aqueous_volume_max_unsupported;
+        eos.configureAqueousVolume(aqueous);
+    }
+    if (d.aqueous_viscosity == NN_AQ_VISCOSITY_IAPWS2008)
+        eos.configureAqueousCompositionDomain(d.water_component,
+                                              d.aqueous_viscosity_max_unsupported);
 
     typename MPMC::FluidSystem<Indices>::ComponentNames names{};
     for (int i = 0; i < N; ++i)
@@ -97,6 +111,12 @@ MPMC::FluidSystem<Indices> makeFluid(const nn_fluid_desc &d, double temperature)
         names, phases, temperature);
     if (d.water_component >= 0)
         fluid.configureFullyCompositionalThreePhase(d.water_component);
+    if (d.aqueous_viscosity == NN_AQ_VISCOSITY_MCBRIDE_WRIGHT)
+        fluid.configureMcBrideWrightAqueousViscosity(d.water_component, d.co2_component,
+                                                     d.aqueous_viscosity_max_unsupported);
+    else if (d.aqueous_viscosity == NN_AQ_VISCOSITY_IAPWS2008)
+        fluid.configureIapws2008AqueousViscosity(d.water_component,
+                                                 d.aqueous_viscosity_max_unsupported);
 
     using Eval = typename Indices::ValueType;
     const double swc = d.relperm[0], sorw = d.relperm[1], sgc = d.relperm[2],

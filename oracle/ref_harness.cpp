@@ -23,8 +23,10 @@
 #include <natural/kernel/cell_kernel.hpp>
 #include <natural/physics/accumulation.hpp>
 #include <natural/physics/face_flux.hpp>
+#include <natural/properties/aqueous_viscosity.hpp>
 #include <natural/petsc/phase_state_codec.hpp>
 #include <natural/petsc/property_conversion.hpp>
+#include <natural/state/newton_limiter.hpp>
 #include <natural/state/state_codec.hpp>
 #include <natural/state/three_phase_equilibrium.hpp>
 #include <natural/thermo/three_phase_flash.hpp>
@@ -545,6 +547,37 @@ int assemblyRun(const Handle &h, const Topology &t, double dt, double fugScale,
     return NN_OK;
 }
 
+// NaturalPetscRuntime::limitNewtonStep (natural_petsc_runtime_state.inc:452-545)
+// restated around the reference's own limiters: delta = -Y, the scaled increment
+// recovered with the variable scales (`params`: the six scaling factors, null =
+// scaling off), limitNaturalNewtonIncrement, limitAqueousCompositionNewtonIncrement
+// when maxAqueous >= 0, then Y = -delta.
+template <int NC, bool Well>
+int newtonLimit(int64_t n, const double *primary, double *step, const double *params,
+                double maxAqueous, int water, int co2)
+{
+    using I = AdIdx<NC, Well>;
+    constexpr int NP = I::numPrimaryVariables;
+    const ScalingRule<I> sc{params, nullptr};
+    for (int64_t c = 0; c < n; ++c)
+    {
+        std::array<double, NP> state{}, delta{};
+        for (int v = 0; v < NP; ++v)
+        {
+            state[v] = primary[c * NP + v];
+            delta[v] = -step[c * NP + v];
+            if (params)
+                delta[v] *= sc.variable(v);
+        }
+        MPMC::limitNaturalNewtonIncrement<I>(delta, state, MPMC::PhasePresence{});
+        if (maxAqueous >= 0.0)
+            MPMC::limitAqueousCompositionNewtonIncrement<I>(delta, state, water, co2, maxAqueous);
+        for (int v = 0; v < NP; ++v)
+            step[c * NP + v] = -delta[v];
+    }
+    return NN_OK;
+}
+
 template <class F>
 int dispatchNc(int nc, F &&f)
 {
@@ -553,7 +586,9 @@ int dispatchNc(int nc, F &&f)
     case 2: return f(std::integral_constant<int, 2>{});
     case 3: return f(std::integral_constant<int, 3>{});
     case 4: return f(std::integral_constant<int, 4>{});
+    case 5: return f(std::integral_constant<int, 5>{});
     case 6: return f(std::integral_constant<int, 6>{});
+    case 8: return f(std::integral_constant<int, 8>{});
     default: return NN_E_UNSUPPORTED;
     }
 }
@@ -575,6 +610,41 @@ void *ref_create(const nn_fluid_desc *desc, const nn_flash_options *opts)
 void ref_destroy(void *h) { delete static_cast<Handle *>(h); }
 
 const char *ref_last_error(void) { return g_error; }
+
+int ref_newton_limit(void *hv, int64_t n, int hasWell, const double *primary, double *step,
+                     const double *scaling, double maxAqueous, int water, int co2)
+{
+    const Handle &h = *static_cast<Handle *>(hv);
+    return dispatchNc(h.desc.nc, [&](auto nc) {
+        constexpr int NC = decltype(nc)::value;
+        return hasWell ? newtonLimit<NC, true>(n, primary, step, scaling, maxAqueous, water, co2)
+                       : newtonLimit<NC, false>(n, primary, step, scaling, maxAqueous, water, co2);
+    });
+}
+
+// The reference's Water-role closures at (P, T) for an H2O/CO2 aqueous phase of
+// CO2 mole fraction xco2: IF97 density (aqueous_volume.hpp:270-295), Garcia CO2
+// volume, IAPWS-2008 and McBride-Wright viscosities; NaN where the reference throws.
+int ref_water_closures(int64_t n, const double *P, const double *T, const double *xco2,
+                       double *density, double *co2Volume, double *muIapws, double *muMbw)
+{
+    const MPMC::Iapws2008IndustrialAqueousViscosity<2> iapws(0, 1.0);
+    const MPMC::McBrideWright2015AqueousViscosity<2> mbw(0, 1, 1.0);
+    for (int64_t i = 0; i < n; ++i)
+    {
+        const std::array<double, 2> x{1.0 - xco2[i], xco2[i]};
+        const std::array<double, 2> pure{1.0, 0.0};
+        auto guarded = [&](auto &&f) {
+            try { return double(f()); }
+            catch (const std::exception &) { return std::nan(""); }
+        };
+        density[i] = guarded([&] { return MPMC::IapwsIf97WaterDensity::density(P[i], T[i]); });
+        co2Volume[i] = MPMC::Garcia2001Co2ApparentVolume::molarVolume(T[i]);
+        muIapws[i] = guarded([&] { return iapws.viscosity(P[i], T[i], pure); });
+        muMbw[i] = guarded([&] { return mbw.viscosity(P[i], T[i], x); });
+    }
+    return NN_OK;
+}
 
 int ref_eos_phase(void *hv, int64_t n, const double *P, const double *T,
                   const double *x, const int32_t *role, double *Z, double *lnphi,

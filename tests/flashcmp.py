@@ -78,3 +78,53 @@ def compare_flash(mine: dict, ref: dict):
         worst_i = int(np.argmax(per))
         worst = float(per[worst_i])
     return n_status, n_label, worst, worst_i, sel
+
+
+def gibbs_energy(ref, P, T, out) -> np.ndarray:
+    """Dimensionless Gibbs energy sum_p beta_p sum_i x_pi (ln x_pi + ln phi_pi) of each
+    flash result, over the slots with beta > 0, with the reference build's EOS
+    (`ref`: refharness.Reference) evaluated for slot p in role p (the role of a
+    canonical slot).  Two converged answers of one state with the same labels whose
+    energies agree are equally valid minima of the same problem."""
+    beta = np.asarray(out["beta"], np.float64)
+    x = np.asarray(out["x"], np.float64)
+    n, _, nc = x.shape
+    idx, slot = np.nonzero(beta > 0.0)
+    g = np.zeros(n)
+    if len(idx) == 0:
+        return g
+    xs = np.maximum(x[idx, slot], 1e-300)
+    _, lnphi, _, status = ref.eos_phase(np.asarray(P, np.float64)[idx], np.asarray(T, np.float64)[idx],
+                                        xs, slot.astype(np.int32))
+    term = beta[idx, slot] * np.sum(xs * (np.log(xs) + lnphi), axis=1)
+    term[status != 0] = np.nan
+    np.add.at(g, idx, term)
+    return g
+
+
+def gibbs_diff(ref, P, T, mine, theirs) -> np.ndarray:
+    """|G(mine) - G(theirs)| / max(1, |G(theirs)|) per state (gibbs_energy)."""
+    a = gibbs_energy(ref, P, T, mine)
+    b = gibbs_energy(ref, P, T, theirs)
+    return np.abs(a - b) / np.maximum(1.0, np.abs(b))
+
+
+def update_as_flash(P, primary, phase_state, nc) -> dict:
+    """The beta / x fields of updated cell states (primary: p, x_o[0..nc-2], x_g[..],
+    x_w[..], ...; phase state: flag, K_go, K_wo, z, beta[3], ...), for gibbs_energy."""
+    primary = np.asarray(primary, np.float64)
+    n = len(primary)
+    x = np.zeros((n, 3, nc))
+    for p in range(3):
+        xs = primary[:, 1 + p * (nc - 1):1 + (p + 1) * (nc - 1)]
+        x[:, p, :nc - 1] = xs
+        x[:, p, nc - 1] = 1.0 - xs.sum(axis=1)
+    beta = np.asarray(phase_state, np.float64)[:, 1 + 3 * nc:4 + 3 * nc]
+    return {"beta": beta, "x": x}
+
+
+def update_gibbs_diff(ref, T, mine_prim, mine_ps, ref_prim, ref_ps, nc) -> np.ndarray:
+    """gibbs_diff of two updated states of the same cells (pressure of `ref_prim`)."""
+    P = np.asarray(ref_prim, np.float64)[:, 0]
+    return gibbs_diff(ref, P, T, update_as_flash(P, mine_prim, mine_ps, nc),
+                      update_as_flash(P, ref_prim, ref_ps, nc))

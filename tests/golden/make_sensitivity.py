@@ -3,7 +3,10 @@ CPA test sets have results conditioned by the math library.
 
 For each state the host build of this library's flash (tests/golden/
 sw_sensitivity.cpp; on the host it reproduces the reference build to ~1e-15)
-is re-run under eight seeded 1-ulp perturbations of exp/log/pow/cos/acos.
+is re-run under eight seeded 1-ulp perturbations of exp/log/pow/cos/acos
+(64 for states that reach the allocation recovery or a near tie, whose
+alternative answers can be rare; the distinct phase-fraction vectors reached
+are recorded as the state's `outcomes`).
 States whose values move by more than 1e-12 (relative, floor 1) — or whose
 label moves — are recorded with that band, and so is every state whose flash
 reaches the Soreide-Whitson exact-mass allocation recovery (a multi-start
@@ -39,6 +42,8 @@ from nonenone_amd import fluids  # noqa: E402
 from nonenone_amd.workloads import batched_flash_states  # noqa: E402
 
 THRESHOLD = 1.0e-12
+SALTS = 8              # perturbed runs per state
+SENSITIVE_SALTS = 64   # per allocation-recovery or near-tie state
 SETS = {
     # name: (fluid factory, source) — golden fixture file or the live seed-7 set
     "flash_nc3_sw": (lambda: fluids.h2o_co2_nc10("SW"), "golden"),
@@ -67,14 +72,23 @@ def states(name, fluid, source):
 
 
 def parse(stdout: str) -> dict:
+    """Listed states: label flips, band above THRESHOLD, allocation recovery, a
+    near tie, or more than one outcome.  `outcomes`: the distinct phase-fraction
+    vectors the perturbed runs that keep the label reached (the unperturbed one
+    first); `band`: the spread of the runs that stay in the first outcome."""
     flagged = {}
     for line in stdout.split("\n"):
         if not line.strip():
             continue
-        idx, _, flips, band, alloc, tie = line.split()
-        if int(flips) > 0 or float(band) > THRESHOLD or int(alloc) or int(tie):
+        f = line.split()
+        idx, _, flips, band, alloc, tie, nout = f[:7]
+        vals = [float(v) for v in f[7:]]
+        outcomes = [vals[3 * k:3 * k + 3] for k in range(int(nout))]
+        if (int(flips) > 0 or float(band) > THRESHOLD or int(alloc) or int(tie) or
+                len(outcomes) > 1):
             flagged[idx] = {"label_flips": int(flips), "band": float(band),
-                            "allocation_recovery": int(alloc), "gibbs_tie": int(tie)}
+                            "allocation_recovery": int(alloc), "gibbs_tie": int(tie),
+                            "outcomes": outcomes}
     return flagged
 
 
@@ -96,34 +110,48 @@ def update_sets(exe: Path, tmp: Path) -> dict:
             np.ascontiguousarray(w["T"], np.float64).tofile(f)
             np.ascontiguousarray(cur, np.float64).tofile(f)
             np.ascontiguousarray(hist_ps, np.float64).tofile(f)
-        res = subprocess.run([str(exe), "update", str(tmp / "desc.bin"), str(tmp / "cells.bin")],
-                             check=True, capture_output=True, text=True)
+        res = subprocess.run([str(exe), "update", str(tmp / "desc.bin"), str(tmp / "cells.bin"),
+                              str(SALTS), str(SENSITIVE_SALTS)], check=True, capture_output=True,
+                             text=True)
         flagged = parse(res.stdout)
         print(f"{name}: {len(flagged)} of {len(cur)} cells libm-conditioned", flush=True)
         out[name] = {"n": int(len(cur)), "conditioned": flagged}
     return out
 
 
+def flash_set(exe: Path, tmp: Path, name: str) -> tuple[str, dict]:
+    factory, source = SETS[name]
+    fluid = factory()
+    P, T, z = states(name, fluid, source)
+    d = tmp / name
+    d.mkdir()
+    (d / "desc.bin").write_bytes(bytes(fluid))
+    with open(d / "states.bin", "wb") as f:
+        np.array([len(P)], np.int64).tofile(f)
+        np.ascontiguousarray(P, np.float64).tofile(f)
+        np.ascontiguousarray(T, np.float64).tofile(f)
+        np.ascontiguousarray(z, np.float64).tofile(f)
+    res = subprocess.run([str(exe), "flash", str(d / "desc.bin"), str(d / "states.bin"),
+                          str(SALTS), str(SENSITIVE_SALTS)], check=True, capture_output=True,
+                         text=True)
+    flagged = parse(res.stdout)
+    print(f"{name}: {len(flagged)} of {len(P)} states libm-conditioned", flush=True)
+    return name, {"n": int(len(P)), "conditioned": flagged}
+
+
 def main() -> None:
-    out = {"threshold": THRESHOLD, "sets": {}}
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = {"threshold": THRESHOLD, "salts": SALTS, "sensitive_salts": SENSITIVE_SALTS, "sets": {}}
     with tempfile.TemporaryDirectory() as td:
         tmp = Path(td)
         exe = build(tmp)
-        out["sets"].update(update_sets(exe, tmp))
-        for name, (factory, source) in SETS.items():
-            fluid = factory()
-            P, T, z = states(name, fluid, source)
-            (tmp / "desc.bin").write_bytes(bytes(fluid))
-            with open(tmp / "states.bin", "wb") as f:
-                np.array([len(P)], np.int64).tofile(f)
-                np.ascontiguousarray(P, np.float64).tofile(f)
-                np.ascontiguousarray(T, np.float64).tofile(f)
-                np.ascontiguousarray(z, np.float64).tofile(f)
-            res = subprocess.run([str(exe), "flash", str(tmp / "desc.bin"), str(tmp / "states.bin")],
-                                 check=True, capture_output=True, text=True)
-            flagged = parse(res.stdout)
-            out["sets"][name] = {"n": int(len(P)), "conditioned": flagged}
-            print(f"{name}: {len(flagged)} of {len(P)} states libm-conditioned", flush=True)
+        with ThreadPoolExecutor(max_workers=len(SETS)) as pool:
+            futures = [pool.submit(flash_set, exe, tmp, name) for name in SETS]
+            out["sets"].update(update_sets(exe, tmp))
+            for fut in futures:
+                name, entry = fut.result()
+                out["sets"][name] = entry
     (HERE / "libm_sensitivity.json").write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
 
 

@@ -10,17 +10,23 @@
 // can legitimately give values that differ there by more than the parity
 // tolerance.  Output, one line per state:
 //     <index> <presence> <label_flips> <worst_relative_change> <allocation_recovery> <near_tie>
+//     <n_outcomes> <beta of outcome 1 (3 values)> ... <beta of outcome n>
 // where allocation_recovery is 1 when the unperturbed flash reached the
 // Soreide-Whitson exact-mass allocation recovery (allocation.inc:609-700) and
 // near_tie is 1 when one of its lowest-Gibbs selections compared two
 // candidates whose reduced Gibbs energies agree to 1e-12 (equivalent
-// solutions, chosen by the last bits of the arithmetic).
+// solutions, chosen by the last bits of the arithmetic).  Such states get
+// `sensitive_salts` perturbed runs instead of `salts` (the alternative answers
+// can be rare), and the distinct phase-fraction vectors of the runs that keep
+// the unperturbed label are listed as outcomes (the first is the unperturbed
+// one; at most kMaxOutcomes).  worst_relative_change is taken over the runs
+// that stay in the unperturbed outcome (the spread of that answer).
 //
 // The same for the hysteretic phase update (phase_update.hpp), with the
 // primary vector compared instead of the flash record.
 //
-// Usage: sw_sensitivity flash <fluid_desc.bin> <states.bin> [salts]
-//        sw_sensitivity update <fluid_desc.bin> <cells.bin> [salts]
+// Usage: sw_sensitivity flash <fluid_desc.bin> <states.bin> [salts [sensitive_salts]]
+//        sw_sensitivity update <fluid_desc.bin> <cells.bin> [salts [sensitive_salts]]
 //   fluid_desc.bin: raw nn_fluid_desc; states.bin: int64 n, then P[n], T[n], z[n][nc];
 //   cells.bin: int64 n, int64 has_well, then T[n], primary[n][NP], phase_state[n][NPS].
 #include <hip/hip_runtime.h>
@@ -31,7 +37,35 @@
 
 static uint64_t g_calls = 0, g_salt = 0;
 static int g_mode = 0;
-static int g_salts = 8;  // perturbed re-runs per state
+static int g_salts = 8;            // perturbed re-runs per state
+static int g_sensitiveSalts = 64;  // for allocation-recovery and near-tie states
+constexpr int kMaxOutcomes = 8;
+
+// Distinct phase-fraction vectors (max abs difference above 1e-9).
+struct Outcomes
+{
+    int n = 0;
+    double beta[kMaxOutcomes][3];
+    // index of the outcome b belongs to (appended when new; -1 when full)
+    int add(const double *b)
+    {
+        for (int k = 0; k < n; ++k)
+            if (std::fabs(beta[k][0] - b[0]) <= 1e-9 && std::fabs(beta[k][1] - b[1]) <= 1e-9 &&
+                std::fabs(beta[k][2] - b[2]) <= 1e-9)
+                return k;
+        if (n == kMaxOutcomes)
+            return -1;
+        for (int p = 0; p < 3; ++p) beta[n][p] = b[p];
+        return n++;
+    }
+    void print() const
+    {
+        std::printf(" %d", n);
+        for (int k = 0; k < n; ++k)
+            std::printf(" %.17g %.17g %.17g", beta[k][0], beta[k][1], beta[k][2]);
+        std::printf("\n");
+    }
+};
 static inline double bump(double r)
 {
     if (!g_mode)
@@ -103,7 +137,11 @@ static int run(const FluidParams &fp, const FlashOptions &op, int64_t n, const d
         const FlashResult<NC> r0 = f0.flash(P[s], T[s], z + s * NC);
         int flips = 0;
         double worst = 0.0;
-        for (int salt = 1; salt <= g_salts; ++salt)
+        Outcomes out;
+        out.add(r0.beta);
+        const int salts =
+            (f0.allocationFallbackReached || f0.nearTie) ? g_sensitiveSalts : g_salts;
+        for (int salt = 1; salt <= salts; ++salt)
         {
             g_mode = 1;
             g_salt = salt;
@@ -116,10 +154,14 @@ static int run(const FluidParams &fp, const FlashOptions &op, int64_t n, const d
                 ++flips;
                 continue;
             }
-            worst = std::fmax(worst, change(r0, r1));
+            // the band is the spread within the unperturbed answer's outcome;
+            // runs that land on another split are listed as outcomes instead
+            if (out.add(r1.beta) == 0)
+                worst = std::fmax(worst, change(r0, r1));
         }
-        std::printf("%lld %d %d %.6e %d %d\n", (long long)s, (int)r0.presence, flips, worst,
+        std::printf("%lld %d %d %.6e %d %d", (long long)s, (int)r0.presence, flips, worst,
                     int(f0.allocationFallbackReached), int(f0.nearTie));
+        out.print();
     }
     return 0;
 }
@@ -138,7 +180,9 @@ static int runUpdate(const FluidParams &fp, const FlashOptions &op, int64_t n, F
         double p0[L::NP], p1[L::NP];
         int alloc = 0, tie = 0, flips = 0, label0 = -1;
         double worst = 0.0;
-        for (int salt = 0; salt <= g_salts; ++salt)
+        Outcomes out;
+        int salts = g_salts;
+        for (int salt = 0; salt <= salts; ++salt)
         {
             g_mode = salt > 0;
             g_salt = salt;
@@ -158,6 +202,9 @@ static int runUpdate(const FluidParams &fp, const FlashOptions &op, int64_t n, F
                 label0 = label;
                 alloc = int(up.flash.allocationFallbackReached);
                 tie = int(up.flash.nearTie);
+                if (alloc || tie)
+                    salts = g_sensitiveSalts;
+                out.add(st.beta);
                 continue;
             }
             if (label != label0)
@@ -165,6 +212,8 @@ static int runUpdate(const FluidParams &fp, const FlashOptions &op, int64_t n, F
                 ++flips;
                 continue;
             }
+            if (out.add(st.beta) != 0)
+                continue;
             for (int k = 0; k < L::NP; ++k)
             {
                 if (p0[k] == p1[k])
@@ -173,17 +222,20 @@ static int runUpdate(const FluidParams &fp, const FlashOptions &op, int64_t n, F
                 worst = std::fmax(worst, std::fabs(p0[k] - p1[k]) / sc);
             }
         }
-        std::printf("%lld %d %d %.6e %d %d\n", (long long)s, label0, flips, worst, alloc, tie);
+        std::printf("%lld %d %d %.6e %d %d", (long long)s, label0, flips, worst, alloc, tie);
+        out.print();
     }
     return 0;
 }
 
 int main(int argc, char **argv)
 {
-    if (argc != 4 && argc != 5)
+    if (argc < 4 || argc > 6)
         return 2;
-    if (argc == 5)
+    if (argc >= 5)
         g_salts = std::atoi(argv[4]);
+    if (argc == 6)
+        g_sensitiveSalts = std::atoi(argv[5]);
     const bool update = std::strcmp(argv[1], "update") == 0;
     nn_fluid_desc d;
     FILE *fd = std::fopen(argv[2], "rb");

@@ -47,6 +47,12 @@ inline FluidParams paramsFrom(const nn_fluid_desc &d)
     p.cpaMaxIter = d.cpa_max_iterations;
     p.cpaTol = d.cpa_tolerance;
     p.cpaDamping = d.cpa_damping;
+    p.aqVolume = d.aqueous_volume;
+    p.aqViscosity = d.aqueous_viscosity;
+    p.co2 = d.co2_component;
+    p.waterMolarMass = d.water_molar_mass;
+    p.aqVolumeMax = d.aqueous_volume_max_unsupported;
+    p.aqViscosityMax = d.aqueous_viscosity_max_unsupported;
     return p;
 }
 

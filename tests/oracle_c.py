@@ -28,6 +28,18 @@ def lib():
         L.oracle_rr2.restype = ctypes.c_int
         L.oracle_lbc_viscosity.argtypes = [P, P, ctypes.c_double, ctypes.c_double]
         L.oracle_lbc_viscosity.restype = ctypes.c_double
+        D = ctypes.c_double
+        for name, args in (("oracle_if97_density", [D, D]),
+                           ("oracle_if97_region3_pressure", [D, D]),
+                           ("oracle_if97_volume", [ctypes.c_int, D, D]),
+                           ("oracle_iapws2008_viscosity", [D, D]),
+                           ("oracle_mcbride_wright_viscosity", [D, D, D]),
+                           ("oracle_garcia_co2_volume", [D])):
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = D
+        L.oracle_newton_limit.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P,
+                                          D, ctypes.c_int, ctypes.c_int]
+        L.oracle_newton_limit.restype = None
         _lib = L
     return _lib
 
@@ -61,3 +73,14 @@ def rr2(z, K, beta0=0.5, tol=1e-12, maxit=80):
 def lbc(fluid, x, rho, T):
     x = np.ascontiguousarray(x, np.float64)
     return lib().oracle_lbc_viscosity(ctypes.byref(fluid), _p(x), float(rho), float(T))
+
+
+def newton_limit(nc, has_well, primary, step, var_scale=None, max_aqueous=None, water=0, co2=1):
+    """The oracle's restatement of limitNewtonStep; returns the limited step."""
+    primary = np.ascontiguousarray(primary, np.float64)
+    out = np.array(step, np.float64, copy=True, order="C")
+    vs = None if var_scale is None else np.ascontiguousarray(var_scale, np.float64)
+    lib().oracle_newton_limit(nc, int(has_well), len(primary), _p(primary), _p(out),
+                              ctypes.c_void_p(0) if vs is None else _p(vs),
+                              -1.0 if max_aqueous is None else float(max_aqueous), water, co2)
+    return out

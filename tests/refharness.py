@@ -40,12 +40,24 @@ def lib():
         L.ref_assembly_scaled.argtypes = ([_P, _I64, ctypes.c_int] + [_P] * 9 +
                                           [ctypes.c_double, ctypes.c_double] + [_P] * 9 +
                                           [ctypes.c_int])
+        L.ref_water_closures.argtypes = [_I64] + [_P] * 7
+        L.ref_newton_limit.argtypes = [_P, _I64, ctypes.c_int, _P, _P, _P, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
 
 def _p(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def water_closures(P, T, xco2):
+    """The reference's IF97 density, Garcia CO2 volume, IAPWS-2008 and McBride-Wright
+    viscosities (NaN where it throws)."""
+    P, T, xco2 = (np.ascontiguousarray(a, np.float64) for a in (P, T, xco2))
+    out = [np.zeros(len(P)) for _ in range(4)]
+    lib().ref_water_closures(len(P), _p(P), _p(T), _p(xco2), *(_p(o) for o in out))
+    return out
 
 
 class Reference:
@@ -109,6 +121,20 @@ class Reference:
         self.L.ref_stability(self.h, n, _p(P), _p(T), _p(z), _p(active), _p(x),
                              _p(out["stable"]), _p(out["valid"]), _p(out["unstable_mask"]),
                              _p(out["trial_sum"]), _p(out["incipient"]), _p(out["status"]))
+        return out
+
+    def newton_limit(self, primary, step, has_well=False, scaling=None, max_aqueous=None,
+                     water=0, co2=1):
+        """limitNewtonStep (restated driver around the reference's limiters) on host
+        arrays; returns the limited step.  scaling: the six factors or None."""
+        primary = np.ascontiguousarray(primary, np.float64)
+        out = np.array(step, np.float64, copy=True, order="C")
+        sc = None if scaling is None else np.ascontiguousarray(scaling, np.float64)
+        rc = self.L.ref_newton_limit(self.h, len(primary), int(has_well), _p(primary), _p(out),
+                                     _p(sc), -1.0 if max_aqueous is None else float(max_aqueous),
+                                     int(water), int(co2))
+        if rc != 0:
+            raise RuntimeError(f"ref_newton_limit failed: {rc}")
         return out
 
     def phase_update(self, T, primary, phase_state, has_well=False, threads=1):

@@ -16,6 +16,7 @@ from nonenone_amd import fluids
 from nonenone_amd._abi import FlashOptions
 
 import assemblycmp
+import flashcmp
 import libm_conditioning
 import refharness
 
@@ -95,17 +96,20 @@ def test_gpu_assembly_matches_fixture():
 @pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("model,has_well", [("PR", False), ("SW", False), ("PR", True),
                                             ("CPA", False), ("CPA", True), ("PR6", False),
-                                            ("SW6", False)])
+                                            ("SW6", False), ("PR5", False), ("PR8", False),
+                                            ("PR8", True), ("CPAaq", True), ("PRaq", False),
+                                            ("PRiapws", True)])
 def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     import torch
     from nonenone_amd.assembly import NaturalAssembly
 
     from nonenone_amd.flash import ThreePhaseFlash
 
-    if model.endswith("6"):
-        fluid = fluids.six_component(model[:-1])
+    many = {"5": fluids.five_component, "6": fluids.six_component, "8": fluids.eight_component}
+    if model[-1] in many:
+        fluid = many[model[-1]](model[:-1])
     else:
-        fluid = fluids.cpa_h2o_co2_nc10() if model == "CPA" else fluids.h2o_co2_nc10(model)
+        fluid = _fluid(model)
     ref = refharness.Reference(fluid, FlashOptions.defaults())
     w, hist_p, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 60, 20, 21, ref,
                                                                 has_well=has_well)
@@ -126,7 +130,9 @@ def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     unexplained, conditioned = libm_conditioning.split_value_failures(
         d.max(axis=1), np.ones(len(d), bool), 1e-10,
         {"SW": "update_sw_60x20_seed21", "SW6": "update_sw6_60x20_seed21"}.get(model, "")
-        if not has_well else "")
+        if not has_well else "",
+        gibbs=flashcmp.update_gibbs_diff(ref, w["T"], m_prim, m_ps, r_prim, r_ps, fluid.nc),
+        beta=m_ps[:, 1 + 3 * fluid.nc:4 + 3 * fluid.nc])
     print(f"phase-update primary max rel diff {d.max():.3e}; above tolerance: {len(conditioned)} "
           f"libm-conditioned {conditioned}, {len(unexplained)} other {unexplained}")
     assert not unexplained
@@ -225,26 +231,18 @@ def test_gpu_newton_assembly_step_replays_in_a_hip_graph():
         _ = (prim * 2.0).sum().item()      # eager work between replays
 
 
-@pytest.mark.gpu
-def test_gpu_assembly_refuses_more_than_six_components():
-    """The Natural assembly is built for up to six components (an eight-component
-    fluid gets flash and phase updates only): creation fails with NN_E_UNSUPPORTED
-    before any kernel runs."""
-    from nonenone_amd.assembly import NaturalAssembly
-    from nonenone_amd.flash import ThreePhaseFlash
-
-    _, topo = load_fixture()
-    n = len(topo["conn_offsets"]) - 1
-    ctx = ThreePhaseFlash(fluids.eight_component("PR"))
-    with pytest.raises(RuntimeError, match="at most 6 components"):
-        NaturalAssembly(ctx, topo, np.ones(n), np.full(n, 0.2), np.full(n, 350.0))
-
-
 # ------------------------------------------------------------------ 3-D grids
 GRID3D = (10, 8, 6)   # nx, ny, nz: Z faces and gravity on every interior k
 
 
 def _fluid(model):
+    """PR / SW / CPA H2O/CO2/nC10, PR6 / SW6 six-component; a suffix "aq" switches
+    on the Water-role closures of the reference's benchmark case (IAPWS-IF97 +
+    Garcia volume, McBride-Wright viscosity), "iapws" the IAPWS-2008 viscosity."""
+    if model.endswith("aq"):
+        return fluids.with_aqueous_closures(_fluid(model[:-2]))
+    if model.endswith("iapws"):
+        return fluids.with_aqueous_closures(_fluid(model[:-5]), viscosity="iapws2008")
     if model.endswith("6"):
         return fluids.six_component(model[:-1])
     return fluids.cpa_h2o_co2_nc10() if model == "CPA" else fluids.h2o_co2_nc10(model)
@@ -396,7 +394,8 @@ def test_central_difference_check_catches_planted_mistakes():
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
-@pytest.mark.parametrize("model,has_well", [("PR", False), ("CPA", True), ("SW", False)])
+@pytest.mark.parametrize("model,has_well", [("PR", False), ("CPA", True), ("SW", False),
+                                            ("CPAaq", True)])
 def test_gpu_3d_phase_update_and_assembly_match_reference_build(model, has_well):
     """10 x 8 x 6 grid with hydrostatic pressure: Z faces with gravity on every
     interior k.  Phase update, accumulation, residual, BSR Jacobian and upwind

@@ -143,6 +143,14 @@ def test_gpu_eos_core_test_goldens():
     np.testing.assert_allclose(fug[1], CORE_FV, rtol=2e-8)
 
 
+def _gibbs(fluid, P, T, mine, ref):
+    """Per-state Gibbs-energy difference of two flash answers (reference build's EOS),
+    or None without the reference build."""
+    if not refharness.available():
+        return None
+    return flashcmp.gibbs_diff(refharness.Reference(fluid, FlashOptions.defaults()), P, T, mine, ref)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_gpu_flash_matches_golden(name):
@@ -151,7 +159,8 @@ def test_gpu_flash_matches_golden(name):
     mine = flashcmp.to_numpy(fl.flash(g["P"], g["T"], g["z"]))
     n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, g)
     unexplained, conditioned = libm_conditioning.split_value_failures(
-        flashcmp.per_state_value_diff(mine, g), sel, flashcmp.REL_TOL, name)
+        flashcmp.per_state_value_diff(mine, g), sel, flashcmp.REL_TOL, name,
+        gibbs=_gibbs(CASES[name](), g["P"], g["T"], mine, g), beta=mine["beta"])
     print(f"{name}: status mismatches {n_status}, label mismatches {n_label}, "
           f"max rel diff {worst:.3e} (state {wi}), compared {int(sel.sum())}; above tolerance: "
           f"{len(conditioned)} libm-conditioned {conditioned}, {len(unexplained)} other {unexplained}")
@@ -172,7 +181,7 @@ def test_gpu_flash_matches_reference_build(name):
     n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, ref)
     unexplained, conditioned = libm_conditioning.split_value_failures(
         flashcmp.per_state_value_diff(mine, ref), sel, flashcmp.REL_TOL,
-        name.replace("flash_", "live_"))
+        name.replace("flash_", "live_"), gibbs=_gibbs(fluid, P, T, mine, ref), beta=mine["beta"])
     print(f"{name}: n={n} status mismatches {n_status}, label mismatches {n_label}, "
           f"max rel diff {worst:.3e}; above tolerance: {len(conditioned)} libm-conditioned "
           f"{conditioned}, {len(unexplained)} other {unexplained}")
@@ -293,7 +302,8 @@ def test_gpu_cooperative_pass_matches_golden(name, monkeypatch):
     mine = flashcmp.to_numpy(fl.flash(g["P"], g["T"], g["z"]))
     n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, g)
     unexplained, conditioned = libm_conditioning.split_value_failures(
-        flashcmp.per_state_value_diff(mine, g), sel, flashcmp.REL_TOL, name)
+        flashcmp.per_state_value_diff(mine, g), sel, flashcmp.REL_TOL, name,
+        gibbs=_gibbs(CASES[name](), g["P"], g["T"], mine, g), beta=mine["beta"])
     print(f"{name} (cooperative pass): status mismatches {n_status}, label mismatches {n_label}, "
           f"max rel diff {worst:.3e}; above tolerance: {len(conditioned)} libm-conditioned, "
           f"{len(unexplained)} other {unexplained}")
@@ -374,11 +384,10 @@ def test_host_build_of_cpa_matches_general_fixture(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["flash_nc3_pr", "flash_nc3_sw", "flash_nc3_cpa"])
+@pytest.mark.parametrize("name", ["flash_nc3_pr", "flash_nc3_sw", "flash_nc3_cpa", "flash_nc6_pr"])
 def test_gpu_plain_form_is_bit_identical_to_scheduled(name):
-    """The plain one-kernel baseline form (one work-item per state, index order)
-    gives the scheduled two-pass form's results bit for bit; it is built for three
-    components and refused for six."""
+    """The plain baseline form (one work-item per state in index order, no
+    persistent queue, no sort) gives the scheduled form's results bit for bit."""
     import torch
 
     fluid = CASES[name]()
@@ -391,6 +400,26 @@ def test_gpu_plain_form_is_bit_identical_to_scheduled(name):
     for key in ("presence", "status", "beta", "x", "Z"):
         assert np.array_equal(np.asarray(plain[key]).view(np.uint8),
                               np.asarray(sched[key]).view(np.uint8)), key
-    six = _flash_gpu(fluids.six_component("PR"))
-    with pytest.raises(RuntimeError, match="three components"):
-        six.set_schedule(plain=1)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("viscosity", ["mcbride_wright", "iapws2008"])
+def test_gpu_flash_with_aqueous_closures_matches_reference_build(viscosity):
+    """H2O/CO2/nC10 PR with the Water-role closures of the reference's benchmark case
+    (IAPWS-IF97 + Garcia molar volume; the IAPWS-2008 variant also restricts the
+    Water role to <= 1e-4 solute): labels, statuses, saturations and molar
+    densities against the reference build."""
+    fluid = fluids.with_aqueous_closures(fluids.h2o_co2_nc10("PR"), viscosity=viscosity)
+    P, T, z = batched_flash_states(1024, fluid, seed=7, near_critical_fraction=0.1)
+    ref = refharness.Reference(fluid, FlashOptions.defaults()).flash(P, T, z, threads=8)
+    mine = flashcmp.to_numpy(_flash_gpu(fluid).flash(P, T, z))
+    n_status, n_label, worst, wi, sel = flashcmp.compare_flash(mine, ref)
+    water = (ref["presence"] & 4) != 0
+    print(f"aqueous closures ({viscosity}): status mismatches {n_status}, label mismatches "
+          f"{n_label}, max rel diff {worst:.3e} (state {wi}); {int(water.sum())} states with water")
+    assert n_status == 0 and n_label == 0
+    assert worst <= flashcmp.REL_TOL
+    # the closure is in force: the water phase's molar density is not P / (R T Z)
+    cubic = P[water] / (8.31446261815324 * T[water] * ref["Z"][water, 2])
+    assert np.min(np.abs(ref["rho_m"][water, 2] / cubic - 1.0)) > 1e-3

@@ -66,3 +66,22 @@ def test_gpu_server_step_matches_direct_calls():
         assert got[0] == checksum(out["residual"])
         assert got[1] == checksum(st)
         assert got[3] == checksum(out["bsr"])
+
+
+@pytest.mark.gpu
+def test_gpu_component_count_mix_is_the_same_alone_and_in_the_mix():
+    """The serving mix of component counts 3, 5, 6 and 8 (PR, SW and PR-CPA
+    members, two members per merged fluid group): every step's residual,
+    Jacobian and statuses are identical whether it runs alone or batched."""
+    from nonenone_amd.serving import Member, Server, run_trace
+    spec = [("nc8-PR", 20, 16), ("nc5-PR", 24, 20), ("nc3-CPA", 16, 12), ("nc8-SW", 16, 12),
+            ("nc5-SW", 20, 14), ("nc6-PR", 18, 18), ("nc3-SW", 30, 20), ("nc8-PR", 12, 10),
+            ("nc5-PR", 14, 12)]
+    members = [Member(kind, nx, ny, seed=300 + i) for i, (kind, nx, ny) in enumerate(spec)]
+    arrivals = [(0.0, m) for _ in range(2) for m in range(len(members))]
+    alone = run_trace(Server(1, 8 << 30), members, arrivals, hold_start=True)
+    mixed = run_trace(Server(16, 8 << 30), members, arrivals, hold_start=True)
+    assert mixed.stats["peak_batch_steps"] > 1, mixed.stats
+    assert mixed.stats["peak_merged_cells"] > 0, mixed.stats
+    assert dict(alone.checksums) == dict(mixed.checksums)
+    assert alone.nonok == mixed.nonok
