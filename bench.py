@@ -545,7 +545,7 @@ def cpa_phase_workload(args) -> dict:
     }
 
 
-SERVING_KINDS = ("nc3-PR", "nc5-SW", "nc6-PR", "nc3-CPA", "nc3-SW", "nc5-PR", "nc6-SW")
+SERVING_KINDS = ("nc3-PR", "nc6-PR", "nc3-CPA", "nc3-SW", "nc6-SW")
 
 
 def serving_workload(args) -> dict:

@@ -140,7 +140,10 @@ typedef struct nn_context nn_context;
 
 /* Creates a context on the current HIP device: uploads the fluid to device
  * constant storage.  Replaces constructing CubicEquationOfState +
- * CubicThreePhaseFlash (three_phase_flash.hpp:47-81).  Synchronous. */
+ * CubicThreePhaseFlash (three_phase_flash.hpp:47-81).  Synchronous.
+ * Returns NN_E_UNSUPPORTED for nc = 5 and nc = 8: those device builds are
+ * switched off (a GPU memory fault of the five-component phase update whose
+ * cause is not identified). */
 int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts,
                       nn_context **out);
 int nn_context_destroy(nn_context *ctx);
@@ -315,7 +318,9 @@ int nn_stability_batch(nn_context *ctx, int64_t n,
  * d_T: per-cell temperature.  d_status: NN_STATUS_* (RECOVERABLE_FAILURE =
  * inputs restored).  d_flags: bit0 phaseRemoved, bit1 stabilityInvalid,
  * bit2 missingPhaseUnstable, bit3 restrictedFlashFailed,
- * bit4 unrestrictedFlashFailed, bit5 stableReducedSet (may be NULL).
+ * bit4 unrestrictedFlashFailed, bit5 stableReducedSet (may be NULL); 0 for a
+ * cell whose status is THERMO_ERROR or INVALID_INPUT (the reference throws and
+ * returns no report).
  */
 int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown,
                           const double *d_T, double *d_primary,
@@ -390,7 +395,7 @@ typedef struct nn_assembly nn_assembly;
 /* Uploads topology (device pointers, kept by reference, must outlive the
  * handle) and precomputes, per connection, the BSR slot of its reciprocal block
  * and, per cell, the slot of its diagonal block.  Synchronous.  Any component
- * count up to NN_MAX_NC. */
+ * count a context accepts (2, 3, 4 and 6). */
 int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
                        const int64_t *d_conn_offsets, const int64_t *d_conn_neighbor,
                        const double *d_conn_trans, const double *d_conn_gdz,

@@ -286,8 +286,12 @@ __host__ __device__ bool aqueousDomain(const FluidParams &f, const double *x)
 }
 
 // IAPWS-Garcia molar volume of the Water role (aqueous_volume.hpp:408-436).
+// Out of line, like aqueousViscosity: the phase-update and flash kernels call the
+// molar density at many sites, and inlining the IF97 evaluation into each would
+// multiply the code of kernels that are already at the register limit; without
+// a closure configured only the not-taken call remains.
 template <int NC, class S>
-__host__ __device__ S aqueousMolarVolume(const FluidParams &f, const S &P, const S &T,
+__host__ __device__ __noinline__ S aqueousMolarVolume(const FluidParams &f, const S &P, const S &T,
                                          const S *x, Err &err)
 {
     if (sval(aqueousUnsupported<NC, S>(f, x, f.co2)) > f.aqVolumeMax)
@@ -364,7 +368,7 @@ __host__ __device__ S iapws2008Viscosity(const S &density, const S &T, Err &err)
 
 // Water-role viscosity (fluid_system.hpp:125-131), Pa s.
 template <int NC, class S>
-__host__ __device__ S aqueousViscosity(const FluidParams &f, const S &P, const S &T, const S *x,
+__host__ __device__ __noinline__ S aqueousViscosity(const FluidParams &f, const S &P, const S &T, const S *x,
                                        Err &err)
 {
     if (f.aqViscosity == NN_AQ_VISCOSITY_IAPWS2008)

@@ -573,6 +573,12 @@ int nn_context_create(const nn_fluid_desc *fluid, const nn_flash_options *opts, 
     *out = nullptr;
     if (fluid->nc < 2 || fluid->nc > NN_MAX_NC)
         return fail(NN_E_INVALID_ARGUMENT, "nc out of range");
+    // The five- and eight-component kernels are switched off: the five-component
+    // phase update + assembly made a GPU memory fault whose cause in this code is
+    // not identified (a host build of the same kernel bodies on the same inputs is
+    // clean under ASan/UBSan and matches the reference bit for bit).
+    if (fluid->nc == 5 || fluid->nc == 8)
+        return fail(NN_E_UNSUPPORTED, "the five- and eight-component device builds are switched off");
     if (fluid->eos_type != 1 && fluid->eos_type != 5)
         return fail(NN_E_INVALID_ARGUMENT, "eos_type must be 1 or 5");
     if (!(fluid->omega_a > 0.0) || !(fluid->omega_b > 0.0) || fluid->m1 == fluid->m2)

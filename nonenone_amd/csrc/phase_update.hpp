@@ -644,9 +644,10 @@ __host__ __device__ int phaseUpdateCell(const FluidParams &f, const FlashOptions
         return kItemDone;
     if (err.bad())
     {
-        // The reference throws out of updatePhaseState; inputs stay untouched.
+        // The reference throws out of updatePhaseState: inputs stay untouched and
+        // there is no PhaseUpdateReport, so no flags either.
         status[s] = err.code;
-        if (flags) flags[s] = fl;
+        if (flags) flags[s] = 0;
         return kItemDone;
     }
     encodePhaseState<NC, Well>(ps, psv);

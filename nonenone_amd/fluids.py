@@ -88,12 +88,14 @@ def make_desc(names, kij, model: str = "PR", water: str = "H2O",
 
 
 def with_aqueous_closures(d: FluidDesc, viscosity: str = "mcbride_wright",
-                          volume: bool = True, co2: int = 1) -> FluidDesc:
+                          volume: bool = True, co2: int = 1,
+                          viscosity_max_unsupported: float = 1.0e-4) -> FluidDesc:
     """Switch on the Water-role closures of the reference's benchmark case
     (case/h2o_co2_nc10_2d_benchmark/case_config.hpp:70-74): IAPWS-IF97 + Garcia
     molar volume and McBride-Wright (or IAPWS-2008, `viscosity="iapws2008"`)
     viscosity, water molar mass 0.018015268 kg/mol, unsupported-solute limits
-    1e-4.  `co2`: index of CO2 (1 in the presets)."""
+    1e-4 (the viscosity closure's: `viscosity_max_unsupported`; IAPWS-2008 counts
+    every non-H2O fraction, CO2 included).  `co2`: index of CO2 (1 in the presets)."""
     from ._abi import NN_AQ_VISCOSITY_IAPWS2008, NN_AQ_VISCOSITY_MCBRIDE_WRIGHT, NN_AQ_VISCOSITY_NONE
     d.aqueous_volume = 1 if volume else 0
     d.aqueous_viscosity = {"mcbride_wright": NN_AQ_VISCOSITY_MCBRIDE_WRIGHT,
@@ -102,7 +104,7 @@ def with_aqueous_closures(d: FluidDesc, viscosity: str = "mcbride_wright",
     d.co2_component = co2
     d.water_molar_mass = 0.018015268
     d.aqueous_volume_max_unsupported = 1.0e-4
-    d.aqueous_viscosity_max_unsupported = 1.0e-4
+    d.aqueous_viscosity_max_unsupported = float(viscosity_max_unsupported)
     return d
 
 

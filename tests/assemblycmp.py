@@ -50,7 +50,13 @@ def reference_grid_states(fluid, nx, ny, seed, ref, has_well=False, nz=1, hydros
     """Seeded nx x ny x nz grid (dx = dy = 10 m, dz = 5 m; with `hydrostatic`, P gains
     rho_w g depth down the k index), history state from the reference's flash, then
     the seeded Newton-like perturbation.  Host arrays: (workload dict, history
-    primary, history phase state, perturbed primary)."""
+    primary, history phase state, perturbed primary).
+
+    With the Water-role closures configured (fluid.aqueous_volume), the Water
+    composition block of the perturbation goes through the reference's
+    aqueous-domain Newton limiter (limitAqueousCompositionNewtonIncrement,
+    newton_limiter.hpp:181-245) at the closures' ceiling, as the runtime limits
+    its Newton steps, so the perturbed states stay inside the closures' domain."""
     from nonenone_amd.assembly import assign_flash_result, perturb_newton_state
     from nonenone_amd.workloads import grid_workload
 
@@ -66,4 +72,11 @@ def reference_grid_states(fluid, nx, ny, seed, ref, has_well=False, nz=1, hydros
         w["P"][c] = w["P"][donor]
     hist_p, hist_ps = assign_flash_result(fl, w["P"], has_well=has_well)
     cur = perturb_newton_state(hist_p, hist_ps, fluid.nc, seed + 1)
+    if fluid.aqueous_volume:
+        nc = fluid.nc
+        limited = ref.newton_limit(hist_p, hist_p - cur, has_well=has_well,
+                                   max_aqueous=fluid.aqueous_volume_max_unsupported,
+                                   water=fluid.water_component, co2=fluid.co2_component)
+        block = slice(1 + 2 * (nc - 1), 1 + 3 * (nc - 1))
+        cur[:, block] = hist_p[:, block] - limited[:, block]
     return w, hist_p, hist_ps, cur

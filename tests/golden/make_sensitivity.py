@@ -1,5 +1,6 @@
 """Writes tests/golden/libm_sensitivity.json: which flash states of the SW and
-CPA test sets have results conditioned by the math library.
+CPA test sets, and which cells of the SW and closure phase-update grids, have
+results conditioned by the math library.
 
 For each state the host build of this library's flash (tests/golden/
 sw_sensitivity.cpp; on the host it reproduces the reference build to ~1e-15)
@@ -21,7 +22,7 @@ solutions, one reached by Newton to ~1e-13 and the other by SSI stopped at the
 only up to the host libm; the GPU tests compare their status and labels
 exactly and report, without asserting, their value differences.
 
-    python tests/golden/make_sensitivity.py
+    python tests/golden/make_sensitivity.py [set names: recompute only those]
 """
 from __future__ import annotations
 
@@ -92,21 +93,37 @@ def parse(stdout: str) -> dict:
     return flagged
 
 
-def update_sets(exe: Path, tmp: Path) -> dict:
-    """The phase-update inputs of the SW grid tests (60 x 20, seed 21, no well):
-    three and six components."""
+UPDATE_SETS = {
+    # name: (fluid factory, well unknown) — the phase-update inputs of the grid
+    # tests (60 x 20, seed 21)
+    "update_sw_60x20_seed21": (lambda: fluids.h2o_co2_nc10("SW"), False),
+    "update_sw6_60x20_seed21": (lambda: fluids.six_component("SW"), False),
+    # PR with the Water-role closures (IAPWS-IF97 + Garcia volume; the viscosity
+    # closure does not enter the phase update)
+    "update_praq_60x20_seed21": (
+        lambda: fluids.with_aqueous_closures(fluids.h2o_co2_nc10("PR")), False),
+    "update_praq_60x20_seed21_well": (
+        lambda: fluids.with_aqueous_closures(fluids.h2o_co2_nc10("PR")), True),
+}
+
+
+def update_sets(exe: Path, tmp: Path, only=None) -> dict:
+    """The phase-update sets of UPDATE_SETS (all, or the names in `only`)."""
     import assemblycmp
     import refharness
     from nonenone_amd._abi import FlashOptions
 
     out = {}
-    for name, fluid in (("update_sw_60x20_seed21", fluids.h2o_co2_nc10("SW")),
-                        ("update_sw6_60x20_seed21", fluids.six_component("SW"))):
+    for name, (factory, has_well) in UPDATE_SETS.items():
+        if only is not None and name not in only:
+            continue
+        fluid = factory()
         ref = refharness.Reference(fluid, FlashOptions.defaults())
-        w, _, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 60, 20, 21, ref)
+        w, _, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 60, 20, 21, ref,
+                                                               has_well=has_well)
         (tmp / "desc.bin").write_bytes(bytes(fluid))
         with open(tmp / "cells.bin", "wb") as f:
-            np.array([len(cur), 0], np.int64).tofile(f)
+            np.array([len(cur), int(has_well)], np.int64).tofile(f)
             np.ascontiguousarray(w["T"], np.float64).tofile(f)
             np.ascontiguousarray(cur, np.float64).tofile(f)
             np.ascontiguousarray(hist_ps, np.float64).tofile(f)
@@ -140,19 +157,25 @@ def flash_set(exe: Path, tmp: Path, name: str) -> tuple[str, dict]:
 
 
 def main() -> None:
+    """All sets, or (arguments) only the named sets, merged into the existing file."""
     from concurrent.futures import ThreadPoolExecutor
 
+    only = set(sys.argv[1:]) or None
+    path = HERE / "libm_sensitivity.json"
     out = {"threshold": THRESHOLD, "salts": SALTS, "sensitive_salts": SENSITIVE_SALTS, "sets": {}}
+    if only is not None:
+        out["sets"] = json.loads(path.read_text())["sets"]
+    names = [n for n in SETS if only is None or n in only]
     with tempfile.TemporaryDirectory() as td:
         tmp = Path(td)
         exe = build(tmp)
-        with ThreadPoolExecutor(max_workers=len(SETS)) as pool:
-            futures = [pool.submit(flash_set, exe, tmp, name) for name in SETS]
-            out["sets"].update(update_sets(exe, tmp))
+        with ThreadPoolExecutor(max_workers=max(1, len(names))) as pool:
+            futures = [pool.submit(flash_set, exe, tmp, name) for name in names]
+            out["sets"].update(update_sets(exe, tmp, only))
             for fut in futures:
                 name, entry = fut.result()
                 out["sets"][name] = entry
-    (HERE / "libm_sensitivity.json").write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
+    path.write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
 
 
 if __name__ == "__main__":

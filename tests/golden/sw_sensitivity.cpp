@@ -81,6 +81,9 @@ static double noisyLog(double x) { return bump(std::log(x)); }
 static double noisyPow(double x, double y) { return bump(std::pow(x, y)); }
 static double noisyCos(double x) { return bump(std::cos(x)); }
 static double noisyAcos(double x) { return bump(std::acos(x)); }
+// the fluid set-up (host_params.hpp: LBC constants) uses the unperturbed libm,
+// as the library's does
+#include "../host_params.hpp"
 #define exp noisyExp
 #define log noisyLog
 #define pow noisyPow
@@ -88,7 +91,6 @@ static double noisyAcos(double x) { return bump(std::acos(x)); }
 #define acos noisyAcos
 #include "../../nonenone_amd/csrc/flash.hpp"
 #include "../../nonenone_amd/csrc/phase_update.hpp"
-#include "../host_params.hpp"
 #undef exp
 #undef log
 #undef pow

