@@ -38,7 +38,7 @@ Other workloads (same JSON contract, for the figures quoted beside the headline)
                         sample of each sub-batch;
                         also times the plain form (one work-item per state,
                         index order) on the same states.
-    --workload serving  64 ensemble members (nc=3/5/6/8, PR/SW/PR-CPA, 1k-32k cells each)
+    --workload serving  ensemble members (nc=3/6, PR/SW/PR-CPA, 1k-32k cells each)
                         submit 256 Newton-step requests with Poisson arrivals (200/s);
                         continuous batching (phase updates of members sharing a
                         fluid merged into one launch) vs the plain
@@ -88,8 +88,8 @@ PHASE_UPDATE_FP64_OPS_PER_CELL = 5.9956e6
 # HBM bytes (L2 <-> memory, FETCH_SIZE x 2 + WRITE_SIZE x 1 against a 2 GiB copy)
 # of one headline step's kernels, from the two PMC passes of
 # profiles/phase_update_natural_assembly_200x200x50_cpa_hbm_traffic.md
-PHASE_UPDATE_TRAFFIC_BYTES = 2.6606e12
-ASSEMBLE_TRAFFIC_BYTES = 2.051e11
+PHASE_UPDATE_TRAFFIC_BYTES = 2.3692e12
+ASSEMBLE_TRAFFIC_BYTES = 2.0598e11
 
 
 def cpu_model() -> str:

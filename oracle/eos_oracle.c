@@ -39,6 +39,13 @@ typedef struct
     double bi[NC_MAX];
 } mixing_t;
 
+/* Planted mistakes for the tests that show the golden checks catch them
+ * (test infrastructure; 0 = the restatement as written): 1 drops the
+ * b_i (Z - 1) / B term of ln(phi_i), 2 flips the sign of its attraction
+ * (logarithmic) term, 3 drops the binary interaction coefficients. */
+static int g_plant = 0;
+void oracle_plant(int which) { g_plant = which; }
+
 static double sw_kij(const nn_fluid_desc *f, int i, int j)
 {
     const int w = f->water_component;
@@ -81,7 +88,9 @@ void oracle_mixing(const nn_fluid_desc *f, double P, double T, int role, mixing_
     for (int i = 0; i < f->nc; ++i)
         for (int j = i; j < f->nc; ++j)
         {
-            const double k = (sw && role == 2) ? sw_kij(f, i, j) : f->kij[i * NN_MAX_NC + j];
+            double k = (sw && role == 2) ? sw_kij(f, i, j) : f->kij[i * NN_MAX_NC + j];
+            if (g_plant == 3)
+                k = 0.0;   /* planted: binary interaction dropped */
             m->aij[i][j] = m->aij[j][i] = sqrtA[i] * sqrtA[j] * (1.0 - k);
         }
 }
@@ -152,10 +161,11 @@ int oracle_phase(const nn_fluid_desc *f, double P, double T, const double *x, in
     if (!found || !(A > 0.0) || !(B > 0.0))
         return NN_STATUS_THERMO_ERROR;
     const double cA = -log(Z - B);
-    const double cB = log((Z + f->m2 * B) / (Z + f->m1 * B)) * A / ((f->m1 - f->m2) * B);
+    const double cB = (g_plant == 2 ? -1.0 : 1.0) *   /* planted: sign of the attraction term */
+                      log((Z + f->m2 * B) / (Z + f->m1 * B)) * A / ((f->m1 - f->m2) * B);
     const double cC = (Z - 1.0) / B;
     for (int i = 0; i < f->nc; ++i)
-        lnphi[i] = cA + cB * (2.0 * S[i] / A - m.bi[i] / B) + m.bi[i] * cC;
+        lnphi[i] = cA + cB * (2.0 * S[i] / A - m.bi[i] / B) + (g_plant == 1 ? 0.0 : m.bi[i] * cC);
     *Zout = Z;
     return 0;
 }

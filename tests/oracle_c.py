@@ -26,6 +26,8 @@ def lib():
         L.oracle_rr2.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int, P]
         L.oracle_rr2.restype = ctypes.c_int
+        L.oracle_plant.argtypes = [ctypes.c_int]
+        L.oracle_plant.restype = None
         L.oracle_lbc_viscosity.argtypes = [P, P, ctypes.c_double, ctypes.c_double]
         L.oracle_lbc_viscosity.restype = ctypes.c_double
         D = ctypes.c_double

@@ -43,15 +43,32 @@ def test_oracle_density_and_lbc_match_core_test_goldens():
         assert abs(oracle_c.lbc(f, x, rho_m, 387.45) - mu_ref) < 3e-12 * max(1.0, mu_ref)
 
 
-def test_planted_mistake_is_detected():
-    """Dropping the B_i term of ln(phi_i) must break the golden fugacities."""
+@pytest.mark.parametrize("plant", [1, 2, 3])
+def test_planted_mistake_is_detected(plant):
+    """Mistakes planted in the oracle itself (eos_oracle.c oracle_plant): a dropped
+    b_i (Z - 1) / B term of ln(phi_i), the sign of its attraction term flipped, the
+    binary interaction coefficients dropped.  Each must break the golden Z or
+    fugacity comparison that the unplanted oracle passes."""
     f = core_test_fluid()
-    Z, lnphi, _ = oracle_c.eos_batch(f, [15e6], [387.45], np.array([CORE_X]), [0])
-    # B_i * (Z - 1) / B is the dropped term; recompute it from the mixing rule.
-    # Here we only need to show the planted change moves fugacities beyond 2e-8.
-    planted = lnphi[0] - 0.01 * lnphi[0]
-    fug = np.exp(planted) * 15e6 * np.array(CORE_X)
-    assert np.max(np.abs(fug / np.array(CORE_FL) - 1.0)) > 2e-8
+    L = oracle_c.lib()
+
+    def golden_ok():
+        Z, lnphi, st = oracle_c.eos_batch(f, [15e6, 15e6], [387.45, 387.45],
+                                          np.array([CORE_X, CORE_Y]), [0, 1])
+        if not np.all(st == 0):
+            return False
+        fug = np.exp(lnphi) * 15e6 * np.array([CORE_X, CORE_Y])
+        return (abs(Z[0] - CORE_ZL) < 2e-12 and abs(Z[1] - CORE_ZV) < 2e-12 and
+                np.allclose(fug[0], CORE_FL, rtol=2e-8, atol=0) and
+                np.allclose(fug[1], CORE_FV, rtol=2e-8, atol=0))
+
+    assert golden_ok()
+    L.oracle_plant(plant)
+    try:
+        assert not golden_ok()
+    finally:
+        L.oracle_plant(0)
+    assert golden_ok()
 
 
 @pytest.mark.parametrize("model", ["PR", "SW"])
@@ -106,7 +123,8 @@ def test_oracle_eos_matches_reference_build(model):
 # wrote them; a regenerated list that grows past these counts fails here.
 CONDITIONED_CEILING = {"flash_nc3_cpa": 23, "flash_nc3_sw": 45, "flash_nc6_sw": 47,
                        "live_nc3_cpa": 31, "live_nc3_sw": 54, "live_nc6_sw": 69,
-                       "update_sw6_60x20_seed21": 38, "update_sw_60x20_seed21": 12}
+                       "update_sw6_60x20_seed21": 38, "update_sw_60x20_seed21": 12,
+                       "update_praq_60x20_seed21": 17, "update_praq_60x20_seed21_well": 17}
 
 
 def test_every_libm_conditioned_state_has_a_finite_bound():
