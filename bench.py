@@ -722,6 +722,8 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=50)
     ap.add_argument("--eos", choices=("CPA", "PR", "SW"), default="CPA")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain-form", action="store_true",
+                    help="skip the one plain-form step after the timed steps")
     ap.add_argument("--workload", choices=("assembly", "flash", "learned", "serving", "cpa_phase",
                                            "phase_cell_ops", "assembly_only", "copy"),
                     default="assembly")
@@ -820,11 +822,38 @@ def main() -> int:
                         "instruction counters on single-cell launches (PR-CPA only); traffic "
                         "is L2<->memory bytes of the passes (PMC), mostly per-lane scratch"}
 
+    gpu = {"prim": s["prim"].cpu().numpy(), "ps": s["ps"].cpu().numpy(),
+           "status": s["status"].cpu().numpy(), "flags": s["flags"].cpu().numpy()}
+    plain_form = None
+    if not args.no_plain_form:
+        # the same step in the plain form (nn_schedule.plain: one work-item per cell
+        # in index order, no condition sort, no lockstep stability trials), once
+        sched_out = {k: v.cpu().numpy() for k, v in s["out"].items()}
+        keep = fl.schedule()
+        fl.set_schedule(plain=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_step(fl, asm, s)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
+        fl.set_schedule(**keep)
+        same = (np.array_equal(s["status"].cpu().numpy(), gpu["status"]) and
+                np.array_equal(s["flags"].cpu().numpy(), gpu["flags"]) and
+                np.array_equal(s["prim"].cpu().numpy(), gpu["prim"]) and
+                np.array_equal(s["ps"].cpu().numpy(), gpu["ps"]) and
+                all(np.array_equal(s["out"][k].cpu().numpy(), v, equal_nan=v.dtype.kind == "f")
+                    for k, v in sched_out.items()))
+        plain_form = {"ms_per_step": plain_s * 1e3,
+                      "speedup_of_scheduled": plain_s * 1e3 / (elapsed * 1e3 / args.steps),
+                      "bit_identical": bool(same),
+                      "form": "one work-item per cell in index order (ceil(n/64) blocks), no "
+                              "condition sort, no lockstep stability trials; cooperative pass "
+                              "for the cells past the lane budget; same assembly"}
+        log(f"plain form: {plain_s:.3f}s per step, results bit-identical: {same}")
+
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()
-        gpu = {"prim": s["prim"].cpu().numpy(), "ps": s["ps"].cpu().numpy(),
-               "status": s["status"].cpu().numpy(), "flags": s["flags"].cpu().numpy()}
         cpu = cpu_baseline(fluid, host, (args.nx, args.ny, args.nz), gpu=gpu)
         log(f"cpu baseline {time.time() - t0:.1f}s: {cpu}")
 
@@ -846,6 +875,7 @@ def main() -> int:
         "phase_update_cells": queue,
         "roofline": roofline,
         "roofline_assembly": roofline_assembly,
+        "plain_form": plain_form,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

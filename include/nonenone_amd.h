@@ -264,10 +264,12 @@ int nn_flash_batch_static(nn_context *ctx, int64_t n,
  *                             condition-key order
  *   wave_claim                0: lanes claim single states from the queue;
  *                             1: wavefronts claim tiles of 64 consecutive states
- *   plain                     1: the baseline form for flash batches: one
- *                             work-item per state in index order (no persistent
- *                             queue, no sort, other fields ignored); states past
- *                             the lane budget still go to the cooperative pass
+ *   plain                     1: the baseline form for flash batches and phase
+ *                             updates: one work-item per state or cell in index
+ *                             order (no persistent queue, no sort, no lockstep
+ *                             stability trials; other fields ignored); states
+ *                             past the lane budget still go to the cooperative
+ *                             pass
  */
 typedef struct nn_schedule {
     int32_t persistent_blocks_per_cu;

@@ -1000,15 +1000,20 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
     if (const int rc = ensureList(ctx, n); rc != NN_OK)
         return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const bool ordered = ctx->sched.ordered && n >= kOrderMinItems && n <= kOrderMaxItems &&
-                         !streamCapturing(st);
+    // The plain form (nn_schedule.plain): one work-item per cell in index order
+    // (ceil(n / 64) blocks), no condition sort and no lockstep stability trials;
+    // cells past the lane budget still go to the cooperative pass.
+    const bool plain = ctx->sched.plain != 0;
+    const bool lockstep = ctx->lockstep && !plain;
+    const bool ordered = !plain && ctx->sched.ordered && n >= kOrderMinItems &&
+                         n <= kOrderMaxItems && !streamCapturing(st);
     if (ordered)
         if (const int rc = ensureSort(ctx, n); rc != NN_OK)
             return rc;
     return dispatchNc(ctx->host.nc, [&](auto ncTag) {
         constexpr int NC = decltype(ncTag)::value;
         resetQueueKernel<<<1, 1, 0, st>>>(ctx->dQueue, ctx->coopAll ? 1u : 0u,
-                                          ctx->sched.wave_claim ? 1u : 0u);
+                                          !plain && ctx->sched.wave_claim ? 1u : 0u);
         NN_HIP(hipGetLastError());
         const unsigned int *order = nullptr;
         if (ordered)
@@ -1017,7 +1022,7 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
                                                 d_phase_state, ctx->dKeys[0], ctx->dIdx[0])));
             NN_HIP(sortOrder(ctx, n, st, &order));
         }
-        if (ctx->lockstep)
+        if (lockstep)
             if (const int rc = ensureRecords(ctx, n, sizeof(StabRecord<NC>)); rc != NN_OK)
                 return rc;
         // an ordered batch also sorts its re-flash pass (the sort buffers are free
@@ -1026,12 +1031,14 @@ int nn_phase_update_batch(nn_context *ctx, int64_t n, int32_t has_well_unknown, 
         const ReflashOrder reorder{ctx->dKeys[0], ctx->dIdx[0], reflashSort, &sortArg};
         LaunchTimer t(ctx, NN_KERNEL_PHASE_UPDATE, st);
         t.makeSplit();
-        NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, persistentGrid(ctx, n),
+        const unsigned grid = plain ? static_cast<unsigned>((n + kBlock - 1) / kBlock)
+                                    : persistentGrid(ctx, n);
+        NN_HIP((Launch<NC>::phaseUpdate(has_well_unknown != 0, grid,
                                         coopGrid(ctx), st, ctx->dFluid, ctx->dOpts, n, d_T,
                                         d_primary, d_phase_state, d_status, d_flags, ctx->dQueue,
-                                        ctx->dList, order, ctx->lockstep ? ctx->dPending : nullptr,
-                                        ctx->lockstep ? ctx->dRecords : nullptr,
-                                        ordered && ctx->lockstep ? &reorder : nullptr, t.splitA,
+                                        ctx->dList, order, lockstep ? ctx->dPending : nullptr,
+                                        lockstep ? ctx->dRecords : nullptr,
+                                        ordered && lockstep ? &reorder : nullptr, t.splitA,
                                         t.splitB)));
         t.doneSplit(NN_KERNEL_PHASE_UPDATE_COOP);
         return NN_OK;

@@ -592,3 +592,34 @@ def test_gpu_scaled_assembly_matches_reference_build(model, has_well):
     again = asm.evaluate(DT, dev(r_prim), dev(r_ps), dev(r_acc))
     for k in ("residual", "bsr", "decisions", "status"):
         assert torch.equal(again[k], plain[k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refharness.available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("model,has_well", [("PR", True), ("CPA", True), ("SW", False)])
+def test_gpu_plain_phase_update_is_bit_identical_to_scheduled(model, has_well):
+    """The plain form of the phase update (nn_schedule.plain: one work-item per cell
+    in index order, no condition sort, no lockstep stability trials) against the
+    scheduled form (sorted order above 4,096 cells, lockstep trials, compacted
+    re-flash pass) on an 80x64 grid: statuses, flags, primaries and phase states
+    bit for bit."""
+    import torch
+    from nonenone_amd.flash import ThreePhaseFlash
+
+    fluid = _fluid(model)
+    ref = refharness.Reference(fluid, FlashOptions.defaults())
+    w, hist_p, hist_ps, cur = assemblycmp.reference_grid_states(fluid, 80, 64, 29, ref,
+                                                                has_well=has_well)
+    ctx = ThreePhaseFlash(fluid)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    outs = {}
+    for form in ("scheduled", "plain"):
+        ctx.set_schedule(plain=1 if form == "plain" else 0)
+        prim, ps = dev(cur), dev(hist_ps)
+        status, flags = ctx.update_phase_state(w["T"], prim, ps, has_well)
+        outs[form] = [status.cpu().numpy(), flags.cpu().numpy(), prim.cpu().numpy(),
+                      ps.cpu().numpy()]
+    print(f"{model}: statuses {np.bincount(outs['plain'][0], minlength=5)}")
+    for a, b, name in zip(outs["scheduled"], outs["plain"], ("status", "flags", "primary",
+                                                               "phase state")):
+        assert np.array_equal(a, b), name
