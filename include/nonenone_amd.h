@@ -392,6 +392,33 @@ int nn_bsr_cell_columns(int64_t ncell, int32_t np, const int64_t *conn_offsets,
                         const int64_t *col_blocks, const double *bsr, double *values,
                         int64_t *rows, int64_t *column_offsets);
 
+/* ---- state samples and their CSV text ------------------------------------ */
+
+/* Output sample of a cell-state array on the device, as the reference's
+ * CpGridOutputAccess::sample (output/include/output/petsc/cpgrid_output_access.hpp:
+ * 42-140) takes it from a PETSc Vec: d_out[i * nvar + k] = d_vec[d_cells[i] *
+ * dof_per_cell + d_vars[k]].  d_vec: ncell x dof_per_cell (a primary or
+ * phase-state array); d_cells (nsel, int64) and d_vars (nvar, int32) index it.
+ * An index outside the array writes NaN to its value and sets *d_bad (the
+ * reference throws std::out_of_range); at most INT_MAX values per sample.
+ * Asynchronous on `stream`. */
+int nn_state_sample(const double *d_vec, int64_t ncell, int32_t dof_per_cell,
+                    const int64_t *d_cells, int64_t nsel, const int32_t *d_vars, int32_t nvar,
+                    double *d_out, int32_t *d_bad, void *stream);
+
+/* Host routines (no device work): the text of the reference's sample CSV files
+ * (solution_samples.csv, phase_state_samples.csv and the mass sample files of
+ * OutputWriter, output/include/output/core/writer.hpp:58-93).  The header is
+ * "value_0,...,value_{count-1}\n" (format.hpp:19-29), a row the values printed
+ * with 16 significant digits, comma-separated, "\n"-terminated (format.hpp:
+ * 32-42).  *length receives the text length without the terminating NUL; with
+ * buf NULL only the length is computed, else cap must exceed it.
+ * nn_csv_parse_row reads one such row back (count values into values[cap]). */
+int nn_csv_sample_header(int64_t count, char *buf, int64_t cap, int64_t *length);
+int nn_csv_sample_row(const double *values, int64_t count, char *buf, int64_t cap,
+                      int64_t *length);
+int nn_csv_parse_row(const char *line, double *values, int64_t cap, int64_t *count);
+
 typedef struct nn_assembly nn_assembly;
 
 /* Uploads topology (device pointers, kept by reference, must outlive the

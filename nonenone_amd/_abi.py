@@ -268,6 +268,14 @@ def _declare(L: ctypes.CDLL) -> None:
     L.nn_assembly_info.restype = ctypes.c_int
     L.nn_newton_limit.argtypes = [_P, _P, _P, _D, _I32, _I32, _P]
     L.nn_newton_limit.restype = ctypes.c_int
+    L.nn_state_sample.argtypes = [_P, _I64, _I32, _P, _I64, _P, _I32, _P, _P, _P]
+    L.nn_state_sample.restype = ctypes.c_int
+    L.nn_csv_sample_header.argtypes = [_I64, ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]
+    L.nn_csv_sample_header.restype = ctypes.c_int
+    L.nn_csv_sample_row.argtypes = [_P, _I64, ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]
+    L.nn_csv_sample_row.restype = ctypes.c_int
+    L.nn_csv_parse_row.argtypes = [ctypes.c_char_p, _P, _I64, ctypes.POINTER(_I64)]
+    L.nn_csv_parse_row.restype = ctypes.c_int
     for name in ("nn_context_set_kernel_timing", "nn_context_kernel_time",
                  "nn_context_queue_counts", "nn_context_create", "nn_context_destroy", "nn_eos_phase_batch",
                  "nn_flash_batch", "nn_flash_restricted_batch", "nn_stability_batch",
@@ -297,7 +305,8 @@ EXPORTED_SYMBOLS = (
     "nn_server_release", "nn_server_get_stats", "nn_server_last_error", "nn_server_hold",
     "nn_flash_batch_static", "nn_scaling_default", "nn_assembly_set_scaling",
     "nn_phase_update_transaction", "nn_bsr_cell_columns", "nn_context_queue_counts",
-    "nn_newton_limit",
+    "nn_newton_limit", "nn_state_sample", "nn_csv_sample_header", "nn_csv_sample_row",
+    "nn_csv_parse_row",
 )
 
 # Water-role viscosity closures (nn_fluid_desc.aqueous_viscosity).

@@ -118,6 +118,32 @@ __global__ void newtonLimitKernel(int64_t n, const double *__restrict__ primary,
                               water, co2);
 }
 
+// Output sample gather (cpgrid_output_access.hpp:42-140): value (i, k) is
+// variable vars[k] of cell cells[i]; one thread per value, consecutive threads
+// on consecutive variables of one cell.
+__global__ void stateSampleKernel(const double *__restrict__ vec, int64_t ncell, int dof,
+                                  const int64_t *__restrict__ cells, int64_t nsel,
+                                  const int32_t *__restrict__ vars, int nvar,
+                                  double *__restrict__ out, int *__restrict__ bad)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nsel * nvar)
+        return;
+    const int64_t i = t / nvar;
+    const int k = int(t - i * nvar);
+    const int64_t c = cells[i];
+    const int v = vars[k];
+    if (c < 0 || c >= ncell || v < 0 || v >= dof)
+    {
+        atomicOr(bad, 1);
+        out[t] = __builtin_nan("");
+        return;
+    }
+    out[t] = vec[c * dof + v];
+}
+
+__global__ void clearFlagKernel(int *flag) { *flag = 0; }
+
 __global__ void resetQueueKernel(WorkQueue *q, unsigned int deferAll, unsigned int claim)
 {
     q->head = 0u;
@@ -1217,6 +1243,122 @@ int nn_bsr_cell_columns(int64_t ncell, int32_t np, const int64_t *conn_offsets,
         }
     }
     column_offsets[ncell] = out;
+    return NN_OK;
+}
+
+int nn_state_sample(const double *d_vec, int64_t ncell, int32_t dof_per_cell,
+                    const int64_t *d_cells, int64_t nsel, const int32_t *d_vars, int32_t nvar,
+                    double *d_out, int32_t *d_bad, void *stream)
+{
+    if (!d_vec || ncell < 0 || dof_per_cell <= 0 || nsel < 0 || nvar < 0 || !d_bad ||
+        (nsel > 0 && nvar > 0 && (!d_cells || !d_vars || !d_out)))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    // cpgrid_output_access.hpp:78-84: the sample travels as one MPI int count
+    if (nsel > 0 && nvar > 0 && nsel > int64_t(0x7FFFFFFF) / nvar)
+        return fail(NN_E_INVALID_ARGUMENT, "output sample larger than INT_MAX values");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // the flag is cleared by a kernel, not a memset, so the call can be captured
+    clearFlagKernel<<<1, 1, 0, st>>>(d_bad);
+    NN_HIP(hipGetLastError());
+    const int64_t count = nsel * nvar;
+    if (count == 0)
+        return NN_OK;
+    const unsigned grid = static_cast<unsigned>((count + 255) / 256);
+    stateSampleKernel<<<grid, 256, 0, st>>>(d_vec, ncell, dof_per_cell, d_cells, nsel, d_vars,
+                                            nvar, d_out, d_bad);
+    NN_HIP(hipGetLastError());
+    return NN_OK;
+}
+
+// std::ostream << double with setprecision(16) and the default float field
+// (format.hpp:32-42) prints as printf's %.16g.
+static int64_t csvAppend(char *buf, int64_t cap, int64_t at, const char *text, int64_t len)
+{
+    if (buf && at + len < cap)
+        std::memcpy(buf + at, text, size_t(len));
+    return at + len;
+}
+
+int nn_csv_sample_header(int64_t count, char *buf, int64_t cap, int64_t *length)
+{
+    if (count < 0 || !length || (buf && cap <= 0))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    // format.hpp:19-29: value_0,...,value_{n-1}
+    int64_t at = 0;
+    char item[32];
+    for (int64_t i = 0; i < count; ++i)
+    {
+        const int len = std::snprintf(item, sizeof item, "%svalue_%lld", i ? "," : "",
+                                      static_cast<long long>(i));
+        at = csvAppend(buf, cap, at, item, len);
+    }
+    at = csvAppend(buf, cap, at, "\n", 1);
+    *length = at;
+    if (buf)
+    {
+        if (at >= cap)
+            return fail(NN_E_INVALID_ARGUMENT, "buffer too small");
+        buf[at] = '\0';
+    }
+    return NN_OK;
+}
+
+int nn_csv_sample_row(const double *values, int64_t count, char *buf, int64_t cap,
+                      int64_t *length)
+{
+    if (count < 0 || (count > 0 && !values) || !length || (buf && cap <= 0))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    int64_t at = 0;
+    char item[40];
+    for (int64_t i = 0; i < count; ++i)
+    {
+        const int len = std::snprintf(item, sizeof item, "%s%.16g", i ? "," : "", values[i]);
+        at = csvAppend(buf, cap, at, item, len);
+    }
+    at = csvAppend(buf, cap, at, "\n", 1);
+    *length = at;
+    if (buf)
+    {
+        if (at >= cap)
+            return fail(NN_E_INVALID_ARGUMENT, "buffer too small");
+        buf[at] = '\0';
+    }
+    return NN_OK;
+}
+
+int nn_csv_parse_row(const char *line, double *values, int64_t cap, int64_t *count)
+{
+    if (!line || !count || (cap > 0 && !values))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    int64_t n = 0;
+    const char *p = line;
+    if (*p == '\0' || *p == '\n')
+    {
+        *count = 0;
+        return NN_OK;
+    }
+    for (;;)
+    {
+        char *end = nullptr;
+        const double v = std::strtod(p, &end);
+        if (end == p)
+            return fail(NN_E_INVALID_ARGUMENT, "not a number in a sample row");
+        if (n < cap)
+            values[n] = v;
+        ++n;
+        p = end;
+        if (*p == ',')
+        {
+            ++p;
+            continue;
+        }
+        if (*p == '\n' || *p == '\0' || (*p == '\r' && (p[1] == '\n' || p[1] == '\0')))
+            break;
+        return fail(NN_E_INVALID_ARGUMENT, "malformed sample row");
+    }
+    *count = n;
+    if (n > cap)
+        return fail(NN_E_INVALID_ARGUMENT, "more values than the output holds");
     return NN_OK;
 }
 

@@ -3,11 +3,14 @@
 batched_flash_states: the batched-flash workload — P ~ U(1, 40) MPa,
 T ~ U(300, 600) K, z ~ Dirichlet(1, ..., 1).  A fraction of the states can be
 drawn near the mixture's vapour-liquid critical region (see
-``near_critical_fraction``): those states take z from the same Dirichlet and
-put (P, T) within 2% of a pseudo-critical point estimated with Kay's rule
-(T_pc = sum z_i Tc_i, P_pc = sum z_i Pc_i), the classical cheap estimate that
-the reference's tools/include/tools/phase_diagram.hpp refines with a full
-critical-locus solve.
+``near_critical_fraction``): with a `locus` table (critical_points) they take a
+composition of the table and (P, T) within 2 % of its estimated critical point;
+without one, z from the same Dirichlet and (P, T) within 2 % of Kay's-rule
+pseudo-critical point (T_pc = sum z_i Tc_i, P_pc = sum z_i Pc_i).
+
+critical_points: the reference's envelope-coalescence estimate of a mixture's
+critical point (tools/include/tools/phase_diagram.hpp:633-704, 860-908), with
+any batched flash.
 """
 from __future__ import annotations
 
@@ -78,7 +81,8 @@ def grid_workload(nx: int, ny: int, nz: int, dx: float, dy: float, dz: float, fl
                 ncell=ncell, seed=seed)
 
 
-def batched_flash_states(n: int, fluid, seed: int = 0, near_critical_fraction: float = 0.0):
+def batched_flash_states(n: int, fluid, seed: int = 0, near_critical_fraction: float = 0.0,
+                         locus: dict | None = None):
     rng = np.random.Generator(np.random.PCG64(seed))
     nc = fluid.nc
     P = rng.uniform(1.0e6, 40.0e6, n)
@@ -87,10 +91,101 @@ def batched_flash_states(n: int, fluid, seed: int = 0, near_critical_fraction: f
     m = int(round(near_critical_fraction * n))
     if m > 0:
         idx = rng.choice(n, m, replace=False)
-        tc = np.array([fluid.tc[i] for i in range(nc)])
-        pc = np.array([fluid.pc[i] for i in range(nc)])
-        tpc = z[idx] @ tc
-        ppc = z[idx] @ pc
-        T[idx] = tpc * rng.uniform(0.98, 1.02, m)
-        P[idx] = np.clip(ppc * rng.uniform(0.98, 1.02, m), 1.0e5, None)
+        if locus is not None:
+            ok = np.nonzero(locus["valid"])[0]
+            if len(ok) == 0:
+                raise ValueError("the critical-point table has no valid point")
+            pick = ok[rng.integers(0, len(ok), m)]
+            z[idx] = locus["z"][pick]
+            T[idx] = locus["Tc"][pick] * rng.uniform(0.98, 1.02, m)
+            P[idx] = np.clip(locus["Pc"][pick] * rng.uniform(0.98, 1.02, m), 1.0e5, None)
+        else:
+            tc = np.array([fluid.tc[i] for i in range(nc)])
+            pc = np.array([fluid.pc[i] for i in range(nc)])
+            tpc = z[idx] @ tc
+            ppc = z[idx] @ pc
+            T[idx] = tpc * rng.uniform(0.98, 1.02, m)
+            P[idx] = np.clip(ppc * rng.uniform(0.98, 1.02, m), 1.0e5, None)
     return P, T, z
+
+
+def critical_points(flash, z, temperatures, pressures, refine_iterations=40,
+                    relative_tolerance=1.0e-8, score_tolerance=0.20):
+    """Estimate of each composition's vapour-liquid critical point from batched
+    flashes.
+
+    flash(P, T, z) -> dict with "presence" (n), "status" (n) and "x" (n, 3, nc)
+    as host arrays, for a batch.  The scan and the boundary refinement restate
+    the reference's phase-diagram tool (tools/include/tools/phase_diagram.hpp):
+    for each composition and each temperature of the `temperatures` axis the
+    ascending `pressures` axis is flashed (pressureTemperature), and the last
+    upper boundary of the two-phase region along it, two-phase (code 3) below
+    and liquid (1, a bubble point) or vapour (2, an upper dew point above the
+    critical temperature) above, is refined by bisection on the phase code until
+    the interval is within `relative_tolerance` or `refine_iterations` pass
+    (refineTransitionPressure, :949-993).  Phase codes are the oil/gas presence
+    bits (the water bit is ignored, so water-bearing mixtures have an oil/gas
+    envelope).
+
+    The critical point is where the coexisting phases become identical, so each
+    boundary point is scored by the L1 distance of the oil and gas compositions
+    (oilGasCompositionL1, :453-466) flashed at the interval's two-phase end, and
+    the lowest score over the temperatures gives (Tc, Pc), valid when it is at
+    most `score_tolerance` (the reference's quality gate, :145-147).  (The
+    reference scores the midpoint between a dew and a bubble point plus the
+    envelope's relative width, :672-704; for mixtures whose lower dew branch is
+    still far below the bubble point at the critical temperature, such as CO2
+    with a heavy alkane, that score does not approach zero, and the boundary's
+    composition distance does.)  All flashes of one stage run as one batch.
+    Returns dict(z, Tc, Pc, score, valid)."""
+    z = np.atleast_2d(np.asarray(z, np.float64))
+    Ts = np.asarray(temperatures, np.float64)
+    Ps = np.asarray(pressures, np.float64)
+    nz, nt, npr = len(z), len(Ts), len(Ps)
+
+    def codes(res):
+        code = np.asarray(res["presence"]).astype(np.int64) & 3
+        code[np.asarray(res["status"]) != 0] = -1
+        return code
+
+    iz, it, ip = np.meshgrid(np.arange(nz), np.arange(nt), np.arange(npr), indexing="ij")
+    iz, it, ip = iz.ravel(), it.ravel(), ip.ravel()
+    scan = codes(flash(Ps[ip], Ts[it], z[iz])).reshape(nz, nt, npr)
+    # the highest two-phase -> single-phase step along each pressure line
+    up = (scan[:, :, :-1] == 3) & ((scan[:, :, 1:] == 1) | (scan[:, :, 1:] == 2))
+    last = np.where(up.any(axis=2), npr - 2 - np.argmax(up[:, :, ::-1], axis=2), -1)
+    sel = np.argwhere(last >= 0)
+    out = dict(z=z, Tc=np.full(nz, np.nan), Pc=np.full(nz, np.nan), score=np.full(nz, np.inf),
+               valid=np.zeros(nz, bool))
+    if len(sel) == 0:
+        return out
+    k0 = last[sel[:, 0], sel[:, 1]]
+    single_code = scan[sel[:, 0], sel[:, 1], k0 + 1]
+    p_two, p_single = Ps[k0].copy(), Ps[k0 + 1].copy()
+    active = np.ones(len(sel), bool)
+    for _ in range(refine_iterations):
+        width = np.abs(p_single - p_two)
+        scale = np.maximum(np.maximum(1.0, np.abs(p_single)), np.abs(p_two))
+        active &= ~(width <= relative_tolerance * scale)
+        w = np.nonzero(active)[0]
+        if len(w) == 0:
+            break
+        mid = 0.5 * (p_single[w] + p_two[w])
+        c = codes(flash(mid, Ts[sel[w, 1]], z[sel[w, 0]]))
+        two = c == 3
+        single = c == single_code[w]
+        p_two[w] = np.where(two, mid, p_two[w])
+        p_single[w] = np.where(single, mid, p_single[w])
+        active[w[~two & ~single]] = False   # a failed or other code stops the refinement
+    res = flash(p_two, Ts[sel[:, 1]], z[sel[:, 0]])
+    x = np.asarray(res["x"])
+    l1 = np.abs(x[:, 0, :] - x[:, 1, :]).sum(axis=1)
+    score = np.where(codes(res) == 3, l1, np.inf)
+    for r in range(len(sel)):   # lowest score per composition, first temperature on ties
+        k = sel[r, 0]
+        if score[r] < out["score"][k]:
+            out["score"][k] = score[r]
+            out["Tc"][k] = Ts[sel[r, 1]]
+            out["Pc"][k] = p_two[r]
+    out["valid"] = out["score"] <= score_tolerance
+    return out

@@ -18,6 +18,7 @@
 // Output goes to oracle/_ref/libnn_ref.so (never committed).
 
 #include <indices/indices.hpp>
+#include <output/core/format.hpp>
 #include <natural/assembly/cell_residual.hpp>
 #include <natural/fluid_system.hpp>
 #include <natural/kernel/cell_kernel.hpp>
@@ -41,6 +42,7 @@
 #include <functional>
 #include <stdexcept>
 #include <thread>
+#include <sstream>
 #include <vector>
 
 namespace
@@ -610,6 +612,20 @@ void *ref_create(const nn_fluid_desc *desc, const nn_flash_options *opts)
 void ref_destroy(void *h) { delete static_cast<Handle *>(h); }
 
 const char *ref_last_error(void) { return g_error; }
+
+// The reference's sample CSV text (output/include/output/core/format.hpp:19-42):
+// the header for `count` values, then one row of `values`; returns the text
+// length (written into buf when it fits, NUL-terminated).
+int64_t ref_csv_samples(int64_t count, const double *values, char *buf, int64_t cap)
+{
+    std::ostringstream os;
+    MPMC::writeIndexedCsvHeader(os, static_cast<std::size_t>(count));
+    MPMC::writeCsvValues(os, std::vector<double>(values, values + count));
+    const std::string text = os.str();
+    if (buf && int64_t(text.size()) < cap)
+        std::memcpy(buf, text.c_str(), text.size() + 1);
+    return int64_t(text.size());
+}
 
 int ref_newton_limit(void *hv, int64_t n, int hasWell, const double *primary, double *step,
                      const double *scaling, double maxAqueous, int water, int co2)

@@ -43,12 +43,24 @@ def lib():
         L.ref_water_closures.argtypes = [_I64] + [_P] * 7
         L.ref_newton_limit.argtypes = [_P, _I64, ctypes.c_int, _P, _P, _P, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int]
+        L.ref_csv_samples.argtypes = [_I64, _P, ctypes.c_char_p, _I64]
+        L.ref_csv_samples.restype = _I64
         _lib = L
     return _lib
 
 
 def _p(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def csv_samples(values) -> str:
+    """The reference's sample CSV text (writeIndexedCsvHeader + writeCsvValues) for
+    one row of `values`."""
+    values = np.ascontiguousarray(values, np.float64)
+    n = lib().ref_csv_samples(len(values), _p(values), None, 0)
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    lib().ref_csv_samples(len(values), _p(values), buf, n + 1)
+    return buf.value.decode()
 
 
 def water_closures(P, T, xco2):
