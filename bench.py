@@ -143,7 +143,16 @@ def build_workload(nx, ny, nz, seed=0, eos="CPA"):
     primary, ps = assign_flash_result(flash, w["P"], has_well=True)
     pert = perturb_newton_state(primary, ps, nc, seed=seed + 1)
     dev = torch.device("cuda")
-    asm = NaturalAssembly(fl, w["topo"], w["volume"], w["porosity"], w["T"], has_well_unknown=True)
+    # four bottom-hole-pressure wells completed through every layer (two CO2
+    # injectors, two producers), their couplings added to the cached pattern
+    specs, perf_cells, perf_wi = workloads.vertical_bhp_wells(nx, ny, nz, 10.0, 10.0, 5.0,
+                                                              w["perm"], w["P"], nc=nc)
+    wells = workloads.make_wells(specs, nc)
+    topo = workloads.pattern_add_wells(w["topo"], wells, perf_cells)
+    asm = NaturalAssembly(fl, topo, w["volume"], w["porosity"], w["T"], has_well_unknown=True)
+    asm.set_wells(wells, perf_cells, perf_wi, workloads.SURFACE_DENSITY)
+    log(f"{len(specs)} BHP wells, {len(perf_cells)} perforations; pattern "
+        f"{int(topo['row_offsets'][-1])} blocks")
     hist_p = torch.as_tensor(primary, device=dev)
     hist_s = torch.as_tensor(ps, device=dev)
     acc = asm.accumulation(hist_p, hist_s)
@@ -900,8 +909,9 @@ def main() -> int:
         "config": {"workload": f"natural assembly {args.nx}x{args.ny}x{args.nz} "
                                f"({ncell} cells), nc=3 H2O/CO2/nC10 {EOS_NAMES[args.eos]}, "
                                f"7-point TPFA with gravity, hydrostatic P, well unknown per "
-                               f"cell (N_p=11), phase update + residual + BSR Jacobian "
-                               f"({nnzb} 11x11 blocks)",
+                               f"cell (N_p=11), 4 BHP wells (2 CO2 injectors, 2 producers) "
+                               f"on {args.nz}-cell completions, phase update + residual + BSR "
+                               f"Jacobian ({nnzb} 11x11 blocks)",
                    "eos": EOS_NAMES[args.eos], "cells": ncell,
                    "connections": nconn, "bsr_blocks": nnzb, "n_primary": np_},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

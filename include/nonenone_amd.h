@@ -392,6 +392,39 @@ int nn_bsr_cell_columns(int64_t ncell, int32_t np, const int64_t *conn_offsets,
                         const int64_t *col_blocks, const double *bsr, double *values,
                         int64_t *rows, int64_t *column_offsets);
 
+/* ---- wells ------------------------------------------------------------------
+ * A well of the Natural runtime (well/specification.hpp WellSpecification,
+ * petsc/well_runtime.hpp): Peaceman-type perforation sources in the mass
+ * conservation rows of the perforated cells (physics/well_source.hpp:19-329),
+ * with the bottom-hole pressure the well-pressure unknown of its
+ * representative cell `bhp_cell`, whose well-control row holds the control
+ * equation.  perforation_begin/end index the perforation arrays passed with it
+ * (cell, well index WI).  Only bottom-hole-pressure control is built
+ * (control = NN_WELL_CONTROL_BHP, R_w = p_bhp - target); rate controls are
+ * refused.  Injection fractions are used when a phase flows into the reservoir. */
+#define NN_WELL_INJECTOR 0          /* well/types.hpp:16-20 order */
+#define NN_WELL_PRODUCER 1
+#define NN_WELL_CONTROL_BHP 0
+typedef struct nn_well {
+    int32_t type;
+    int32_t control;
+    int64_t bhp_cell;
+    double target;                                      /* Pa */
+    double injection_phase_fraction[3];                 /* oil, gas, water */
+    double injection_component_mass_fraction[NN_MAX_NC];
+    int64_t perforation_begin;
+    int64_t perforation_end;
+} nn_well;
+
+/* Host routine: the cached BSR pattern with the wells' couplings added
+ * (jacobian_factory.hpp:65-155): the row of each well's bhp_cell gains the
+ * column blocks of its perforated cells, each perforated cell's row the
+ * bhp_cell's column block; rows stay sorted and unique.  Pass NULL outputs to
+ * query the new block count in *out_nnzb. */
+int nn_pattern_add_wells(int64_t ncell, const int64_t *row_offsets, const int64_t *col_blocks,
+                         int32_t nwells, const nn_well *wells, const int64_t *perforation_cells,
+                         int64_t *out_row_offsets, int64_t *out_col_blocks, int64_t *out_nnzb);
+
 /* ---- state samples and their CSV text ------------------------------------ */
 
 /* Output sample of a cell-state array on the device, as the reference's
@@ -504,6 +537,22 @@ int nn_assembly_set_scaling(nn_assembly *a, const nn_scaling *s, const double *d
 int nn_newton_limit(nn_assembly *a, const double *d_primary, double *d_step,
                     double max_aqueous_unsupported, int32_t water_component,
                     int32_t co2_component, void *stream);
+
+/* Wells of the handle's later evaluations (NaturalPetscRuntime::setWells and
+ * the well parts of formFunction / formJacobian: natural_petsc_runtime_wells.inc,
+ * residual.inc:44-121, jacobian.inc:301-457).  Host arrays, copied; nwells = 0
+ * removes them.  Needs a well unknown per cell (has_well_unknown) and a
+ * pattern holding the wells' couplings (nn_pattern_add_wells).  In every later
+ * nn_assembly_evaluate: each perforated cell's mass rows get
+ * R_i += outward flux_i - sum over its perforations of q_i (in well, then
+ * perforation order), each bhp_cell's well-control row R = p_bhp - target, and
+ * the Jacobian gains the perforations' derivatives: columns of the perforated
+ * cell, except the well-pressure column, which goes to the block (cell,
+ * bhp_cell); pattern blocks no face or diagonal term covers are written as
+ * zeros first.  surface_density: oil, gas, water (kg/m3), > 0. */
+int nn_assembly_set_wells(nn_assembly *a, int32_t nwells, const nn_well *wells,
+                          int64_t nperforations, const int64_t *perforation_cells,
+                          const double *perforation_well_index, const double *surface_density);
 
 /* Sizes of an assembly handle: its context, cells, BSR blocks and primary
  * variables per cell (no counterpart in the reference). */

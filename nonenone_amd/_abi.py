@@ -34,6 +34,26 @@ _I64 = ctypes.c_int64
 _P = ctypes.c_void_p
 
 
+class Well(ctypes.Structure):
+    """nn_well."""
+
+    _fields_ = [
+        ("type", _I32),
+        ("control", _I32),
+        ("bhp_cell", _I64),
+        ("target", _D),
+        ("injection_phase_fraction", _D * 3),
+        ("injection_component_mass_fraction", _D * NN_MAX_NC),
+        ("perforation_begin", _I64),
+        ("perforation_end", _I64),
+    ]
+
+
+NN_WELL_INJECTOR = 0
+NN_WELL_PRODUCER = 1
+NN_WELL_CONTROL_BHP = 0
+
+
 class FluidDesc(ctypes.Structure):
     """nn_fluid_desc."""
 
@@ -268,6 +288,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.nn_assembly_info.restype = ctypes.c_int
     L.nn_newton_limit.argtypes = [_P, _P, _P, _D, _I32, _I32, _P]
     L.nn_newton_limit.restype = ctypes.c_int
+    L.nn_pattern_add_wells.argtypes = [_I64, _P, _P, _I32, _P, _P, _P, _P, ctypes.POINTER(_I64)]
+    L.nn_pattern_add_wells.restype = ctypes.c_int
+    L.nn_assembly_set_wells.argtypes = [_P, _I32, _P, _I64, _P, _P, _P]
+    L.nn_assembly_set_wells.restype = ctypes.c_int
     L.nn_state_sample.argtypes = [_P, _I64, _I32, _P, _I64, _P, _I32, _P, _P, _P]
     L.nn_state_sample.restype = ctypes.c_int
     L.nn_csv_sample_header.argtypes = [_I64, ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]
@@ -306,7 +330,7 @@ EXPORTED_SYMBOLS = (
     "nn_flash_batch_static", "nn_scaling_default", "nn_assembly_set_scaling",
     "nn_phase_update_transaction", "nn_bsr_cell_columns", "nn_context_queue_counts",
     "nn_newton_limit", "nn_state_sample", "nn_csv_sample_header", "nn_csv_sample_row",
-    "nn_csv_parse_row",
+    "nn_csv_parse_row", "nn_pattern_add_wells", "nn_assembly_set_wells",
 )
 
 # Water-role viscosity closures (nn_fluid_desc.aqueous_viscosity).

@@ -79,6 +79,18 @@ class NaturalAssembly:
                                                     _ptr(self._well_row_scale)),
                    "nn_assembly_set_scaling")
 
+    def set_wells(self, wells, perforation_cells, perforation_well_index, surface_density):
+        """NaturalPetscRuntime::setWells for the later evaluations (nn_assembly_set_wells):
+        `wells` an nn_well ctypes array (workloads.make_wells), perforation arrays on the
+        host; an empty list removes the wells."""
+        pc = np.ascontiguousarray(perforation_cells, np.int64)
+        wi = np.ascontiguousarray(perforation_well_index, np.float64)
+        sd = np.ascontiguousarray(surface_density, np.float64)
+        p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+        _abi.check(self.lib.nn_assembly_set_wells(
+            self.handle, len(wells), ctypes.cast(wells, ctypes.c_void_p) if len(wells) else None,
+            len(pc), p(pc), p(wi), p(sd)), "nn_assembly_set_wells")
+
     def limit_newton_step(self, primary: torch.Tensor, step: torch.Tensor,
                           max_aqueous_unsupported: float | None = None, water: int = 0,
                           co2: int = 1) -> torch.Tensor:

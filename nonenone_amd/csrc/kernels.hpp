@@ -619,6 +619,7 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
     double roots[9] = {-1.0, 0.0, 0.0, -1.0, 0.0, 0.0, -1.0, 0.0, 0.0};
     if (v.roots)
         for (int k = 0; k < 9; ++k) roots[k] = v.roots[k * v.ncell + c];
+    int32_t wellStatus = NN_STATUS_OK;   // a perforation source the reference rejects
     for (int chunk = 0; chunk < NCHUNK; ++chunk)
     {
         const int col0 = chunk * CH;
@@ -760,9 +761,34 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
             }
         if (chunk == 0 && residual)
         {
+            // the perforations' sources enter as outward flux - source
+            // (addCellConservationTransportAndWell, cell_residual.hpp:435-447), summed in
+            // well then perforation order (natural_petsc_runtime_wells.inc:93-165)
+            double wellSource[NC];
+            bool perforated = false;
+            if constexpr (Well)
+                if (v.wellCellOffsets && v.wellCellOffsets[c + 1] > v.wellCellOffsets[c])
+                {
+                    perforated = true;
+                    for (int i = 0; i < NC; ++i) wellSource[i] = 0.0;
+                    for (int64_t k = v.wellCellOffsets[c]; k < v.wellCellOffsets[c + 1]; ++k)
+                    {
+                        const int32_t perf = v.wellCellPerfs[k];
+                        const DeviceWell &wl = v.wells[v.perfWell[perf]];
+                        const S bhp(primary[wl.bhpCell * NP + L::well]);
+                        S src[NC];
+                        perforationSource<NC, S>(wl, v.perfIndex[perf], v.surfaceDensity, st.p,
+                                                 bhp, pr.rho, pr.mob, pr.w, src, err);
+                        for (int i = 0; i < NC; ++i) wellSource[i] += src[i].v;
+                    }
+                    if (err.bad())
+                        wellStatus = err.code;
+                }
             // natural_petsc_runtime_common.inc:345-356
             for (int i = 0; i < NC; ++i)
-                residual[c * NP + i] = (res[i].v + outward[i]) * rowScale[i];
+                residual[c * NP + i] = perforated
+                    ? (res[i].v + (outward[i] - wellSource[i])) * rowScale[i]
+                    : (res[i].v + outward[i]) * rowScale[i];
             for (int r = NC; r < NP; ++r)
                 residual[c * NP + r] = res[r].v * rowScale[r];
         }
@@ -770,7 +796,96 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
             break;
     }
     if (status)
-        status[c] = NN_STATUS_OK;
+        status[c] = wellStatus;
+}
+
+// insertWellJacobian_ (natural_petsc_runtime_jacobian.inc:301-457) for BHP-controlled
+// wells: one thread per perforated cell adds, perforation by perforation, the
+// derivatives of -source to its mass rows: the cell's own columns into its diagonal
+// block, the well-pressure column into block (cell, bhp cell).  Derivative chunks
+// as in assembleKernel.
+template <int NC, bool Well>
+__global__ void __launch_bounds__(64, kWavesPerEu) wellJacobianKernel(
+    const FluidParams *__restrict__ fp, AssemblyView v, const int64_t *__restrict__ cells,
+    int64_t ncells, const double *__restrict__ primary, const double *__restrict__ phaseState,
+    double *__restrict__ bsr, int32_t *__restrict__ status)
+{
+    using L = Layout<NC, Well>;
+    constexpr int NP = L::NP;
+    constexpr int CH = kAssemblyChunk;
+    constexpr int NCHUNK = (NP + CH - 1) / CH;
+    using S = Dual<CH>;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= ncells)
+        return;
+    const int64_t c = cells[t];
+    const FluidParams &f = *fp;
+    uint8_t presence = 0;
+    if (!loadCell<NC, Well>(f, v, primary, phaseState, c, presence))
+        return;   // the assembly pass has reported this cell
+    double roots[9] = {-1.0, 0.0, 0.0, -1.0, 0.0, 0.0, -1.0, 0.0, 0.0};
+    if (v.roots)
+        for (int k = 0; k < 9; ++k) roots[k] = v.roots[k * v.ncell + c];
+    double *self = bsr + v.selfSlot[c] * NP * NP;
+    for (int chunk = 0; chunk < NCHUNK; ++chunk)
+    {
+        const int col0 = chunk * CH;
+        Err err;
+        CellStateT<NC, S> st;
+        CellPropsT<NC, S> pr;
+        decodeState<NC, Well, S>(primary + c * NP, presence, st, col0);
+        evaluateProperties<NC, S>(f, st, v.porosity[c], v.T[c], pr, err, roots);
+        if (err.bad())
+            return;
+        for (int64_t k = v.wellCellOffsets[c]; k < v.wellCellOffsets[c + 1]; ++k)
+        {
+            const int32_t perf = v.wellCellPerfs[k];
+            const DeviceWell &wl = v.wells[v.perfWell[perf]];
+            // the bottom-hole pressure: the bhp cell's well-pressure unknown
+            S bhp(primary[wl.bhpCell * NP + L::well]);
+            if (L::well >= col0 && L::well < col0 + CH)
+                bhp.d[L::well - col0] = 1.0;
+            S src[NC];
+            perforationSource<NC, S>(wl, v.perfIndex[perf], v.surfaceDensity, st.p, bhp, pr.rho,
+                                     pr.mob, pr.w, src, err);
+            if (err.bad())
+            {
+                if (status) status[c] = err.code;
+                return;
+            }
+            double *coupling = wl.bhpCell == c ? self : bsr + v.perfBhpSlot[perf] * NP * NP;
+            for (int q = 0; q < CH; ++q)
+            {
+                const int col = col0 + q;
+                if (col >= NP)
+                    break;
+                double *block = col == L::well ? coupling : self;
+                for (int i = 0; i < NC; ++i)
+                {
+                    const S eq = -src[i];
+                    block[i * NP + col] += eq.d[q] * v.rowScale[i] * v.varScale[col];
+                }
+            }
+        }
+    }
+}
+
+// overwriteWellControlResiduals_ (natural_petsc_runtime_wells.inc:167-243), BHP
+// control: R_w = p_bhp - target on the bhp cell's well-control row, in well order.
+template <int NC, bool Well>
+__global__ void wellControlKernel(AssemblyView v, int nwells, const double *__restrict__ primary,
+                                  double *__restrict__ residual)
+{
+    using L = Layout<NC, Well>;
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    for (int w = 0; w < nwells; ++w)
+    {
+        const DeviceWell &wl = v.wells[w];
+        const int64_t r = wl.bhpCell;
+        const double scale = v.wellRowScale ? v.wellRowScale[r] : v.rowScale[L::NP - 1];
+        residual[r * L::NP + L::well] = (primary[r * L::NP + L::well] - wl.target) * scale;
+    }
 }
 // ------------------------------------------------------------ launchers
 template <int NC>
@@ -958,6 +1073,36 @@ hipError_t Launch<NC>::assemble(bool well, hipStream_t st, const FluidParams *fp
         assembleKernel<NC, false><<<grid, 64, 0, st>>>(fp, v, dt, fugScale, primary, phaseState,
                                                        accumPrev, residual, bsr, decisions, status);
     return hipGetLastError();
+    }
+}
+
+template <int NC>
+hipError_t Launch<NC>::wells(hipStream_t st, const FluidParams *fp, AssemblyView v, int nwells,
+                             const int64_t *cells, int64_t ncells, const double *primary,
+                             const double *phaseState, double *residual, double *bsr,
+                             int32_t *status)
+{
+    if constexpr (NC > kAssemblyMaxNc)
+    {
+        return hipErrorNotSupported;
+    }
+    else
+    {
+        if (bsr && ncells > 0)
+        {
+            const unsigned grid = static_cast<unsigned>((ncells + 63) / 64);
+            wellJacobianKernel<NC, true><<<grid, 64, 0, st>>>(fp, v, cells, ncells, primary,
+                                                              phaseState, bsr, status);
+            if (const hipError_t e = hipGetLastError(); e != hipSuccess)
+                return e;
+        }
+        if (residual && nwells > 0)
+        {
+            wellControlKernel<NC, true><<<1, 1, 0, st>>>(v, nwells, primary, residual);
+            if (const hipError_t e = hipGetLastError(); e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
     }
 }
 

@@ -10,6 +10,7 @@
 
 #include "../../include/nonenone_amd.h"
 #include "common.hpp"
+#include "wells.hpp"
 
 namespace nn
 {
@@ -68,6 +69,17 @@ struct AssemblyView
     const double *wellRowScale;
     double rowScale[kMaxNp];
     double varScale[kMaxNp];
+    // Wells (nn_assembly_set_wells; wellCellOffsets null when none): per cell the
+    // perforations [wellCellOffsets[c], wellCellOffsets[c + 1]) of wellCellPerfs, in
+    // well then perforation order; per perforation its well, well index and the
+    // BSR slot of block (cell, bhp cell).
+    const int64_t *wellCellOffsets;
+    const int32_t *wellCellPerfs;
+    const int32_t *perfWell;
+    const double *perfIndex;
+    const int64_t *perfBhpSlot;
+    const DeviceWell *wells;
+    double surfaceDensity[3];
 };
 
 // Counters of a two-pass (lane-per-item, then wavefront-per-deferred-item) launch.
@@ -157,6 +169,12 @@ struct Launch
                                const double *phaseState, const double *accumPrev,
                                double *residual, double *bsr, uint8_t *decisions,
                                int32_t *status);
+    // The wells' Jacobian rows (one thread per perforated cell of `cells`) and
+    // well-control residuals, after assemble.
+    static hipError_t wells(hipStream_t st, const FluidParams *fp, AssemblyView v, int nwells,
+                            const int64_t *cells, int64_t ncells, const double *primary,
+                            const double *phaseState, double *residual, double *bsr,
+                            int32_t *status);
 };
 
 } // namespace nn

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -143,6 +144,18 @@ __global__ void stateSampleKernel(const double *__restrict__ vec, int64_t ncell,
 }
 
 __global__ void clearFlagKernel(int *flag) { *flag = 0; }
+
+// Blocks of the pattern that no face or diagonal term writes (the wells'
+// couplings): zeros before the assembly.
+__global__ void zeroBlocksKernel(double *__restrict__ bsr, const int64_t *__restrict__ slots,
+                                 int64_t count, int np)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t per = int64_t(np) * np;
+    if (t >= count * per)
+        return;
+    bsr[slots[t / per] * per + t % per] = 0.0;
+}
 
 __global__ void resetQueueKernel(WorkQueue *q, unsigned int deferAll, unsigned int claim)
 {
@@ -1246,6 +1259,51 @@ int nn_bsr_cell_columns(int64_t ncell, int32_t np, const int64_t *conn_offsets,
     return NN_OK;
 }
 
+int nn_pattern_add_wells(int64_t ncell, const int64_t *row_offsets, const int64_t *col_blocks,
+                         int32_t nwells, const nn_well *wells, const int64_t *perforation_cells,
+                         int64_t *out_row_offsets, int64_t *out_col_blocks, int64_t *out_nnzb)
+{
+    if (ncell < 0 || nwells < 0 || !row_offsets || !col_blocks || !out_nnzb ||
+        (nwells > 0 && (!wells || !perforation_cells)) ||
+        (out_col_blocks && !out_row_offsets))
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    // jacobian_factory.hpp:123-154: bhp row <- perforation blocks; perforation row <- bhp block
+    std::vector<std::vector<int64_t>> extra(static_cast<size_t>(ncell));
+    for (int w = 0; w < nwells; ++w)
+    {
+        const nn_well &wl = wells[w];
+        if (wl.bhp_cell < 0 || wl.bhp_cell >= ncell || wl.perforation_begin < 0 ||
+            wl.perforation_end < wl.perforation_begin)
+            return fail(NN_E_INVALID_ARGUMENT, "invalid well");
+        for (int64_t k = wl.perforation_begin; k < wl.perforation_end; ++k)
+        {
+            const int64_t c = perforation_cells[k];
+            if (c < 0 || c >= ncell)
+                return fail(NN_E_INVALID_ARGUMENT, "perforation cell outside the grid");
+            extra[static_cast<size_t>(wl.bhp_cell)].push_back(c);
+            extra[static_cast<size_t>(c)].push_back(wl.bhp_cell);
+        }
+    }
+    int64_t total = 0;
+    std::vector<int64_t> row;
+    for (int64_t r = 0; r < ncell; ++r)
+    {
+        row.assign(col_blocks + row_offsets[r], col_blocks + row_offsets[r + 1]);
+        row.insert(row.end(), extra[static_cast<size_t>(r)].begin(), extra[static_cast<size_t>(r)].end());
+        std::sort(row.begin(), row.end());
+        row.erase(std::unique(row.begin(), row.end()), row.end());
+        if (out_row_offsets)
+            out_row_offsets[r] = total;
+        if (out_col_blocks)
+            std::copy(row.begin(), row.end(), out_col_blocks + total);
+        total += int64_t(row.size());
+    }
+    if (out_row_offsets)
+        out_row_offsets[ncell] = total;
+    *out_nnzb = total;
+    return NN_OK;
+}
+
 int nn_state_sample(const double *d_vec, int64_t ncell, int32_t dof_per_cell,
                     const int64_t *d_cells, int64_t nsel, const int32_t *d_vars, int32_t nvar,
                     double *d_out, int32_t *d_bad, void *stream)
@@ -1374,7 +1432,46 @@ struct nn_assembly
     int64_t *dRecip = nullptr;
     double *dRecord = nullptr;
     double *dRoots = nullptr;
+    const int64_t *rowOffsetsDevice = nullptr;   // the caller's pattern (kept by reference)
+    const int64_t *colBlocksDevice = nullptr;
+    // pattern blocks no face or diagonal term writes (the wells' couplings)
+    int64_t *dZeroSlots = nullptr;
+    int64_t nZero = 0;
+    // wells (nn_assembly_set_wells)
+    int nWells = 0;
+    int64_t nWellCells = 0;
+    int64_t *dWellCellOffsets = nullptr;
+    int32_t *dWellCellPerfs = nullptr;
+    int32_t *dPerfWell = nullptr;
+    double *dPerfIndex = nullptr;
+    int64_t *dPerfBhpSlot = nullptr;
+    DeviceWell *dWells = nullptr;
+    int64_t *dWellCells = nullptr;
 };
+
+static void releaseWells(nn_assembly *a)
+{
+    for (void *p : {static_cast<void *>(a->dWellCellOffsets), static_cast<void *>(a->dWellCellPerfs),
+                    static_cast<void *>(a->dPerfWell), static_cast<void *>(a->dPerfIndex),
+                    static_cast<void *>(a->dPerfBhpSlot), static_cast<void *>(a->dWells),
+                    static_cast<void *>(a->dWellCells)})
+        if (p) (void)hipFree(p);
+    a->dWellCellOffsets = nullptr;
+    a->dWellCellPerfs = nullptr;
+    a->dPerfWell = nullptr;
+    a->dPerfIndex = nullptr;
+    a->dPerfBhpSlot = nullptr;
+    a->dWells = nullptr;
+    a->dWellCells = nullptr;
+    a->nWells = 0;
+    a->nWellCells = 0;
+    a->view.wellCellOffsets = nullptr;
+    a->view.wellCellPerfs = nullptr;
+    a->view.perfWell = nullptr;
+    a->view.perfIndex = nullptr;
+    a->view.perfBhpSlot = nullptr;
+    a->view.wells = nullptr;
+}
 
 extern "C" {
 
@@ -1398,6 +1495,8 @@ int nn_assembly_destroy(nn_assembly *a)
     if (a->dRecip) (void)hipFree(a->dRecip);
     if (a->dRecord) (void)hipFree(a->dRecord);
     if (a->dRoots) (void)hipFree(a->dRoots);
+    if (a->dZeroSlots) (void)hipFree(a->dZeroSlots);
+    releaseWells(a);
     delete a;
     return NN_OK;
 }
@@ -1449,16 +1548,25 @@ int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
             covered[recip[k]]++;
         }
     }
+    // a block written by two terms would be overwritten; a block no term writes is
+    // a well coupling (nn_pattern_add_wells), zeroed before each assembly
+    std::vector<int64_t> zeroSlots;
     for (int64_t b = 0; b < nnzb; ++b)
-        if (covered[b] != 1)
+    {
+        if (covered[b] > 1)
             return fail(NN_E_INVALID_ARGUMENT,
-                        "every pattern block must be a diagonal or one reciprocal face block");
+                        "a pattern block is both a diagonal and a face block, or two face blocks");
+        if (covered[b] == 0)
+            zeroSlots.push_back(b);
+    }
     nn_assembly *a = new (std::nothrow) nn_assembly;
     if (!a)
         return fail(NN_E_OUT_OF_MEMORY, "host allocation failed");
     a->ctx = ctx;
     a->hasWell = has_well_unknown ? 1 : 0;
     a->nnzb = nnzb;
+    a->rowOffsetsDevice = d_row_offsets;
+    a->colBlocksDevice = d_col_blocks;
     const int nc = ctx->host.nc;
     const size_t recordBytes = sizeof(double) * size_t(10 + 3 * nc) * size_t(ncell);
     const bool cpa = ctx->host.model == NN_MODEL_CPA;
@@ -1469,6 +1577,17 @@ int nn_assembly_create(nn_context *ctx, int64_t ncell, int32_t has_well_unknown,
     {
         nn_assembly_destroy(a);
         return fail(NN_E_OUT_OF_MEMORY, "device allocation failed");
+    }
+    if (!zeroSlots.empty())
+    {
+        if (hipMalloc(&a->dZeroSlots, sizeof(int64_t) * zeroSlots.size()) != hipSuccess)
+        {
+            nn_assembly_destroy(a);
+            return fail(NN_E_OUT_OF_MEMORY, "device allocation failed");
+        }
+        NN_HIP(hipMemcpy(a->dZeroSlots, zeroSlots.data(), sizeof(int64_t) * zeroSlots.size(),
+                         hipMemcpyHostToDevice));
+        a->nZero = int64_t(zeroSlots.size());
     }
     NN_HIP(hipMemcpy(a->dSelf, self.data(), sizeof(int64_t) * ncell, hipMemcpyHostToDevice));
     if (nconn > 0)
@@ -1494,6 +1613,134 @@ void nn_scaling_default(nn_scaling *s)
     s->fugacity_residual_scale = 1.0;
     s->closure_residual_scale = 1.0;
     s->rate_well_residual_floor = 1.0;
+}
+
+int nn_assembly_set_wells(nn_assembly *a, int32_t nwells, const nn_well *wells,
+                          int64_t nperforations, const int64_t *perforation_cells,
+                          const double *perforation_well_index, const double *surface_density)
+{
+    if (!a || nwells < 0 || nperforations < 0)
+        return fail(NN_E_INVALID_ARGUMENT, "invalid argument");
+    if (nwells == 0)
+    {
+        releaseWells(a);
+        return NN_OK;
+    }
+    if (!wells || !surface_density || (nperforations > 0 && (!perforation_cells || !perforation_well_index)))
+        return fail(NN_E_INVALID_ARGUMENT, "null argument");
+    if (!a->hasWell)
+        return fail(NN_E_INVALID_ARGUMENT, "wells need a well unknown per cell");
+    if (nperforations > int64_t(0x7FFFFFFF))
+        return fail(NN_E_INVALID_ARGUMENT, "too many perforations");
+    const int64_t ncell = a->view.ncell;
+    const int nc = a->ctx->host.nc;
+    // well_source.hpp:69-83 and the specification's validation
+    for (int p = 0; p < 3; ++p)
+        if (!(surface_density[p] > 0.0) || !std::isfinite(surface_density[p]))
+            return fail(NN_E_INVALID_ARGUMENT, "surface density must be positive");
+    for (int64_t k = 0; k < nperforations; ++k)
+    {
+        if (perforation_cells[k] < 0 || perforation_cells[k] >= ncell)
+            return fail(NN_E_INVALID_ARGUMENT, "perforation cell outside the grid");
+        if (!(perforation_well_index[k] >= 0.0) || !std::isfinite(perforation_well_index[k]))
+            return fail(NN_E_INVALID_ARGUMENT, "well index must be finite and nonnegative");
+    }
+    std::vector<DeviceWell> dw(static_cast<size_t>(nwells));
+    std::vector<int32_t> perfWell(static_cast<size_t>(nperforations), -1);
+    for (int w = 0; w < nwells; ++w)
+    {
+        const nn_well &wl = wells[w];
+        if (wl.type != NN_WELL_INJECTOR && wl.type != NN_WELL_PRODUCER)
+            return fail(NN_E_INVALID_ARGUMENT, "unknown well type");
+        if (wl.control != NN_WELL_CONTROL_BHP)
+            return fail(NN_E_UNSUPPORTED, "only bottom-hole-pressure control is built");
+        if (wl.bhp_cell < 0 || wl.bhp_cell >= ncell || !std::isfinite(wl.target))
+            return fail(NN_E_INVALID_ARGUMENT, "invalid bhp cell or target");
+        if (wl.perforation_begin < 0 || wl.perforation_end < wl.perforation_begin ||
+            wl.perforation_end > nperforations)
+            return fail(NN_E_INVALID_ARGUMENT, "invalid perforation range");
+        DeviceWell &d = dw[static_cast<size_t>(w)];
+        d.type = wl.type;
+        d.bhpCell = wl.bhp_cell;
+        d.target = wl.target;
+        for (int p = 0; p < 3; ++p) d.injPhase[p] = wl.injection_phase_fraction[p];
+        for (int i = 0; i < kMaxNc; ++i)
+            d.injMass[i] = i < nc ? wl.injection_component_mass_fraction[i] : 0.0;
+        for (int64_t k = wl.perforation_begin; k < wl.perforation_end; ++k)
+        {
+            if (perfWell[static_cast<size_t>(k)] >= 0)
+                return fail(NN_E_INVALID_ARGUMENT, "a perforation belongs to two wells");
+            perfWell[static_cast<size_t>(k)] = w;
+        }
+    }
+    // per-cell perforation lists in well, then perforation order (the order in which
+    // natural_petsc_runtime_wells.inc:93-165 sums a cell's sources)
+    std::vector<int64_t> offsets(static_cast<size_t>(ncell) + 1, 0), bhpSlot(
+        static_cast<size_t>(nperforations), -1);
+    std::vector<int32_t> order;
+    order.reserve(static_cast<size_t>(nperforations));
+    for (int w = 0; w < nwells; ++w)
+        for (int64_t k = wells[w].perforation_begin; k < wells[w].perforation_end; ++k)
+            order.push_back(int32_t(k));
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return perforation_cells[x] < perforation_cells[y];
+    });
+    std::vector<int64_t> cellsWithWells;
+    for (int32_t k : order)
+    {
+        const int64_t c = perforation_cells[k];
+        ++offsets[static_cast<size_t>(c) + 1];
+        if (cellsWithWells.empty() || cellsWithWells.back() != c)
+            cellsWithWells.push_back(c);
+    }
+    for (int64_t c = 0; c < ncell; ++c) offsets[c + 1] += offsets[c];
+    // the block (perforated cell, bhp cell) of each perforation in the pattern
+    std::vector<int64_t> ro(static_cast<size_t>(ncell) + 1), cols(static_cast<size_t>(a->nnzb));
+    NN_HIP(hipMemcpy(ro.data(), a->rowOffsetsDevice, sizeof(int64_t) * (ncell + 1), hipMemcpyDeviceToHost));
+    NN_HIP(hipMemcpy(cols.data(), a->colBlocksDevice, sizeof(int64_t) * a->nnzb, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < nperforations; ++k)
+    {
+        if (perfWell[static_cast<size_t>(k)] < 0)
+            return fail(NN_E_INVALID_ARGUMENT, "a perforation belongs to no well");
+        const int64_t row = perforation_cells[k];
+        const int64_t col = wells[perfWell[static_cast<size_t>(k)]].bhp_cell;
+        const int64_t *b = cols.data() + ro[row], *e = cols.data() + ro[row + 1];
+        const int64_t *it = std::lower_bound(b, e, col);
+        if (it == e || *it != col)
+            return fail(NN_E_INVALID_ARGUMENT,
+                        "the pattern lacks a well coupling block (nn_pattern_add_wells)");
+        bhpSlot[static_cast<size_t>(k)] = int64_t(it - cols.data());
+    }
+    releaseWells(a);
+    auto upload = [](auto **dst, const auto &src) {
+        using T = std::remove_reference_t<decltype(src[0])>;
+        const size_t bytes = sizeof(T) * std::max<size_t>(1, src.size());
+        if (hipMalloc(reinterpret_cast<void **>(dst), bytes) != hipSuccess)
+            return false;
+        return src.empty() ||
+               hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice) ==
+                   hipSuccess;
+    };
+    std::vector<double> wi(perforation_well_index, perforation_well_index + nperforations);
+    if (!upload(&a->dWellCellOffsets, offsets) || !upload(&a->dWellCellPerfs, order) ||
+        !upload(&a->dPerfWell, perfWell) || !upload(&a->dPerfIndex, wi) ||
+        !upload(&a->dPerfBhpSlot, bhpSlot) || !upload(&a->dWells, dw) ||
+        !upload(&a->dWellCells, cellsWithWells))
+    {
+        releaseWells(a);
+        return fail(NN_E_OUT_OF_MEMORY, "device allocation of the wells failed");
+    }
+    a->nWells = nwells;
+    a->nWellCells = int64_t(cellsWithWells.size());
+    AssemblyView &v = a->view;
+    v.wellCellOffsets = a->dWellCellOffsets;
+    v.wellCellPerfs = a->dWellCellPerfs;
+    v.perfWell = a->dPerfWell;
+    v.perfIndex = a->dPerfIndex;
+    v.perfBhpSlot = a->dPerfBhpSlot;
+    v.wells = a->dWells;
+    for (int p = 0; p < 3; ++p) v.surfaceDensity[p] = surface_density[p];
+    return NN_OK;
 }
 
 int nn_assembly_set_scaling(nn_assembly *a, const nn_scaling *s, const double *d_well_row_scale)
@@ -1608,11 +1855,23 @@ int nn_assembly_evaluate(nn_assembly *a, double dt, double fugacity_scaling,
         NN_HIP((Launch<NC>::record(a->hasWell != 0, st, a->ctx->dFluid, v, d_primary, d_phase_state,
                                    nullptr, nullptr)));
         tr.done();
+        if (d_bsr && a->nZero > 0)
+        {
+            const int64_t np = a->hasWell ? 3 * NC + 2 : 3 * NC + 1;
+            const int64_t count = a->nZero * np * np;
+            zeroBlocksKernel<<<static_cast<unsigned>((count + 255) / 256), 256, 0, st>>>(
+                d_bsr, a->dZeroSlots, a->nZero, int(np));
+            NN_HIP(hipGetLastError());
+        }
         LaunchTimer ta(a->ctx, NN_KERNEL_ASSEMBLE, st);
         NN_HIP((Launch<NC>::assemble(a->hasWell != 0, st, a->ctx->dFluid, v, dt, fugacity_scaling,
                                      d_primary, d_phase_state, d_accum_prev, d_residual, d_bsr,
                                      d_decisions, d_status)));
         ta.done();
+        if (a->nWells > 0)
+            NN_HIP((Launch<NC>::wells(st, a->ctx->dFluid, v, a->nWells, a->dWellCells,
+                                      a->nWellCells, d_primary, d_phase_state, d_residual, d_bsr,
+                                      d_status)));
         return NN_OK;
     });
 }

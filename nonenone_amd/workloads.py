@@ -189,3 +189,86 @@ def critical_points(flash, z, temperatures, pressures, refine_iterations=40,
             out["Pc"][k] = p_two[r]
     out["valid"] = out["score"] <= score_tolerance
     return out
+
+
+# Surface densities (oil, gas, water; kg/m3) of case/h2o_co2_nc10_2d_benchmark/case_config.hpp:65.
+SURFACE_DENSITY = (730.0, 1.8, 985.4040020947351)
+
+
+def make_wells(specs, nc):
+    """An nn_well ctypes array from dicts with keys type ("injector" / "producer"),
+    bhp_cell, target (Pa), perforations (range of the perforation arrays) and, for
+    injectors, injection_phase_fraction (3) and injection_component_mass_fraction (nc)."""
+    from ._abi import NN_WELL_CONTROL_BHP, NN_WELL_INJECTOR, NN_WELL_PRODUCER, Well
+    arr = (Well * len(specs))()
+    for w, sp in zip(arr, specs):
+        w.type = NN_WELL_INJECTOR if sp["type"] == "injector" else NN_WELL_PRODUCER
+        w.control = NN_WELL_CONTROL_BHP
+        w.bhp_cell = int(sp["bhp_cell"])
+        w.target = float(sp["target"])
+        for p, v in enumerate(sp.get("injection_phase_fraction", (0.0, 0.0, 0.0))):
+            w.injection_phase_fraction[p] = float(v)
+        for i, v in enumerate(sp.get("injection_component_mass_fraction", (0.0,) * nc)):
+            w.injection_component_mass_fraction[i] = float(v)
+        w.perforation_begin, w.perforation_end = int(sp["perforations"][0]), int(sp["perforations"][1])
+    return arr
+
+
+def vertical_bhp_wells(nx, ny, nz, dx, dy, dz, perm, P, co2=1, nc=3, wells=None):
+    """Bottom-hole-pressure wells completed through every layer of a column: by
+    default two CO2 injectors (gas phase, pure CO2, BHP 2 MPa above the column's top
+    pressure) at (nx/4, ny/4) and (3nx/4, 3ny/4) and two producers (BHP 0.5 MPa
+    below it) at (3nx/4, ny/4) and (nx/4, 3ny/4), the offsets of the reference's
+    benchmark wells (case/h2o_co2_nc10_2d_benchmark/benchmark_common.hpp:53-54).
+    Well index: Peaceman's isotropic form 2 pi k dz / ln(r_eq / r_w),
+    r_eq = 0.14 sqrt(dx^2 + dy^2), r_w = 0.1 m.  The bhp cell is the top
+    perforation.  Returns (specs, perforation_cells, perforation_well_index)."""
+    perm = np.asarray(perm, np.float64)
+    kcell = perm if perm.ndim == 1 else perm[:, 0]
+    if wells is None:
+        wells = [("injector", nx // 4, ny // 4), ("producer", (3 * nx) // 4, ny // 4),
+                 ("injector", (3 * nx) // 4, (3 * ny) // 4), ("producer", nx // 4, (3 * ny) // 4)]
+    req = 0.14 * np.hypot(dx, dy)
+    specs, cells, wis = [], [], []
+    for kind, i, j in wells:
+        col = [i + nx * (j + ny * k) for k in range(nz)]
+        top = col[0]
+        begin = len(cells)
+        cells += col
+        wis += [2.0 * np.pi * kcell[c] * dz / np.log(req / 0.1) for c in col]
+        sp = dict(type=kind, bhp_cell=top, perforations=(begin, len(cells)))
+        if kind == "injector":
+            sp["target"] = float(P[top]) + 2.0e6
+            sp["injection_phase_fraction"] = (0.0, 1.0, 0.0)
+            mf = [0.0] * nc
+            mf[co2] = 1.0
+            sp["injection_component_mass_fraction"] = mf
+        else:
+            sp["target"] = float(P[top]) - 0.5e6
+        specs.append(sp)
+    return specs, np.array(cells, np.int64), np.array(wis)
+
+
+def pattern_add_wells(topo, wells, perforation_cells):
+    """The topology with the wells' couplings added to its cached BSR pattern
+    (nn_pattern_add_wells)."""
+    from ._abi import lib
+    L = lib()
+    n = len(topo["row_offsets"]) - 1
+    pc = np.ascontiguousarray(perforation_cells, np.int64)
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    ro = np.ascontiguousarray(topo["row_offsets"], np.int64)
+    cb = np.ascontiguousarray(topo["col_blocks"], np.int64)
+    nnzb = ctypes.c_int64()
+    w = ctypes.cast(wells, ctypes.c_void_p)
+    if L.nn_pattern_add_wells(n, p(ro), p(cb), len(wells), w, p(pc), None, None,
+                              ctypes.byref(nnzb)) != 0:
+        raise RuntimeError("nn_pattern_add_wells failed")
+    out_ro = np.zeros(n + 1, np.int64)
+    out_cb = np.zeros(nnzb.value, np.int64)
+    if L.nn_pattern_add_wells(n, p(ro), p(cb), len(wells), w, p(pc), p(out_ro), p(out_cb),
+                              ctypes.byref(nnzb)) != 0:
+        raise RuntimeError("nn_pattern_add_wells failed")
+    t = dict(topo)
+    t["row_offsets"], t["col_blocks"] = out_ro, out_cb
+    return t

@@ -26,6 +26,7 @@
 #include <natural/physics/face_flux.hpp>
 #include <natural/properties/aqueous_viscosity.hpp>
 #include <natural/petsc/phase_state_codec.hpp>
+#include <natural/physics/well_source.hpp>
 #include <natural/petsc/property_conversion.hpp>
 #include <natural/state/newton_limiter.hpp>
 #include <natural/state/state_codec.hpp>
@@ -315,6 +316,16 @@ struct Topology
     const double *T;
 };
 
+// Wells of an assembly (nn_well), host arrays; nwells = 0: none.
+struct WellSet
+{
+    int32_t nwells = 0;
+    const nn_well *wells = nullptr;
+    const int64_t *perfCells = nullptr;
+    const double *perfIndex = nullptr;
+    const double *surfaceDensity = nullptr;
+};
+
 int64_t findSlot(const Topology &t, int64_t row, int64_t col)
 {
     const int64_t *b = t.colBlocks + t.rowOffsets[row];
@@ -438,9 +449,98 @@ struct AssemblyRunner
 
     // residual.inc:59-120 and jacobian.inc:20-237 for cells [b, e); each cell
     // writes its residual row, its diagonal block and its reciprocal blocks.
+    // natural_petsc_runtime_wells.inc:54-91: one perforation's source with the
+    // reference's own computePerforationWellSourceInto, bhp the representative
+    // cell's well-pressure unknown (broadcastWellBottomHolePressures_, :8-49).
+    MPMC::PerforationWellResult<I, Scalar> perforation(const WellSet &ws, int32_t well,
+                                                       int64_t k, const double *primary) const
+    {
+        const nn_well &wl = ws.wells[well];
+        const int64_t c = ws.perfCells[k];
+        const Scalar bhp = MPMC::PrimaryVariables<I>::make(
+            primary[wl.bhp_cell * NP + I::Primary::wellPressure], I::Primary::wellPressure);
+        std::array<Scalar, 3> phasePressure{states[c].pressure, states[c].pressure,
+                                            states[c].pressure};
+        std::array<double, 3> surface{ws.surfaceDensity[0], ws.surfaceDensity[1],
+                                      ws.surfaceDensity[2]};
+        std::array<double, 3> injPhase{wl.injection_phase_fraction[0],
+                                       wl.injection_phase_fraction[1],
+                                       wl.injection_phase_fraction[2]};
+        std::array<double, NC> injMass{};
+        for (int i = 0; i < NC; ++i)
+            injMass[i] = wl.injection_component_mass_fraction[i];
+        MPMC::PerforationWellResult<I, Scalar> result;
+        MPMC::PerforationWellWorkspace<I, Scalar> work;
+        MPMC::computePerforationWellSourceInto<I, Scalar>(
+            result, work, wl.type == NN_WELL_INJECTOR ? MPMC::WellType::Injector
+                                                      : MPMC::WellType::Producer,
+            ws.perfIndex[k], bhp, phasePressure, props[c].density, props[c].mobility, surface,
+            injPhase, injMass, props[c].massFraction, props[c].aqueousCO2MassFraction, -1);
+        return result;
+    }
+
+    // evaluateWellFunctionContributions_ (:93-165): per-cell sums in well, then
+    // perforation order
+    std::vector<std::array<Scalar, NC>> cellSources(const WellSet &ws, const double *primary) const
+    {
+        std::vector<std::array<Scalar, NC>> src;
+        if (ws.nwells == 0)
+            return src;
+        src.assign(static_cast<std::size_t>(t.ncell), std::array<Scalar, NC>{});
+        for (int32_t wl = 0; wl < ws.nwells; ++wl)
+            for (int64_t k = ws.wells[wl].perforation_begin; k < ws.wells[wl].perforation_end; ++k)
+            {
+                const auto r = perforation(ws, wl, k, primary);
+                for (int i = 0; i < NC; ++i)
+                    src[static_cast<std::size_t>(ws.perfCells[k])][i] += r.componentMassSource[i];
+            }
+        return src;
+    }
+
+    // insertWellJacobian_ (jacobian.inc:301-457) for active BHP-controlled wells:
+    // each perforation adds its rows, the well-pressure column moved to the
+    // representative cell's block; and overwriteWellControlResiduals_
+    // (wells.inc:167-243): R_w = p_bhp - target on the representative cell's row.
+    void addWells(const WellSet &ws, const ScalingRule<I> &sc, const double *primary,
+                  double *residual, double *bsr) const
+    {
+        constexpr int N = NP;
+        for (int32_t wl = 0; wl < ws.nwells; ++wl)
+        {
+            const nn_well &well = ws.wells[wl];
+            if (residual)
+            {
+                const int row = I::Equation::wellControl;
+                const double bhp = primary[well.bhp_cell * N + I::Primary::wellPressure];
+                residual[well.bhp_cell * N + row] =
+                    (bhp - well.target) * sc.equation(row, well.bhp_cell);
+            }
+            if (!bsr)
+                continue;
+            for (int64_t k = well.perforation_begin; k < well.perforation_end; ++k)
+            {
+                const int64_t c = ws.perfCells[k];
+                const auto r = perforation(ws, wl, k, primary);
+                for (int q = 0; q < NC; ++q)
+                {
+                    const int row = I::Equation::massConservation[q];
+                    const Scalar eq = -r.componentMassSource[q];
+                    const double rowScale = sc.equation(row, c);
+                    for (int col = 0; col < N; ++col)
+                    {
+                        const int64_t colBlock = col == I::Primary::wellPressure ? well.bhp_cell : c;
+                        bsr[findSlot(t, c, colBlock) * N * N + row * N + col] +=
+                            eq.derivative(col) * rowScale * sc.variable(col);
+                    }
+                }
+            }
+        }
+    }
+
     void evaluate(Worker &w, double dt, double fugScale, const ScalingRule<I> &sc,
                   const double *accumPrev, double *residual, double *bsr,
-                  uint8_t *decisionsOut, int64_t b, int64_t e)
+                  uint8_t *decisionsOut, int64_t b, int64_t e,
+                  const std::vector<std::array<Scalar, NC>> *wellSource = nullptr)
     {
         constexpr int N = NP;
         std::vector<MPMC::FaceFluxLinearizationDecision<I>> decisions;
@@ -477,7 +577,9 @@ struct AssemblyRunner
                 auto withFlux = block;
                 std::array<Scalar, NC> noWell{};
                 MPMC::addCellConservationTransportAndWell<I>(
-                    withFlux, outward, noWell, Scalar(0.0));
+                    withFlux, outward,
+                    wellSource ? (*wellSource)[static_cast<std::size_t>(c)] : noWell,
+                    Scalar(0.0));
                 // natural_petsc_runtime_common.inc:345-356
                 for (int r = 0; r < N; ++r)
                     residual[c * N + r] = MPMC::scalarValue(withFlux.value[r]) * sc.equation(r, c);
@@ -525,7 +627,8 @@ int assemblyRun(const Handle &h, const Topology &t, double dt, double fugScale,
                 const double *scaling, const double *wellRowScale,
                 const double *primary, const double *phaseState,
                 const double *accumPrev, double *residual, double *bsr,
-                uint8_t *decisions, double *accumOut, int threads)
+                uint8_t *decisions, double *accumOut, int threads,
+                const WellSet &wells = WellSet{})
 {
     using Runner = AssemblyRunner<NC, Well>;
     using I = typename Runner::I;
@@ -541,11 +644,18 @@ int assemblyRun(const Handle &h, const Topology &t, double dt, double fugScale,
     });
     if (accumOut)
         runner.accumulation(accumOut, 0, t.ncell);
+    if (wells.nwells > 0 && !Well)
+        throw std::invalid_argument("wells need a well unknown per cell");
+    const auto wellSource = runner.cellSources(wells, primary);
     if (residual || bsr)
         parallelFor(t.ncell, threads, [&](int, int64_t b, int64_t e) {
             typename Runner::Worker w(h.desc);
-            runner.evaluate(w, dt, fugScale, sc, accumPrev, residual, bsr, decisions, b, e);
+            runner.evaluate(w, dt, fugScale, sc, accumPrev, residual, bsr, decisions, b, e,
+                            wellSource.empty() ? nullptr : &wellSource);
         });
+    if constexpr (Well)
+        if (wells.nwells > 0)
+            runner.addWells(wells, sc, primary, residual, bsr);
     return NN_OK;
 }
 
@@ -737,6 +847,42 @@ int ref_assembly_scaled(void *hv, int64_t ncell, int hasWell, const int64_t *con
                 : assemblyRun<NC, false>(h, t, dt, fugScale, scaling, wellRowScale, primary,
                                          phaseState, accumPrev, residual, bsr, decisions,
                                          accumOut, threads);
+        });
+    }
+    catch (const std::exception &e)
+    {
+        std::snprintf(g_error, sizeof(g_error), "%s", e.what());
+        return NN_E_INVALID_ARGUMENT;
+    }
+}
+
+int ref_assembly_wells(void *hv, int64_t ncell, int hasWell, const int64_t *connOffsets,
+                       const int64_t *connNeighbor, const double *connTrans,
+                       const double *connGdz, const int64_t *rowOffsets,
+                       const int64_t *colBlocks, const double *volume,
+                       const double *porosity, const double *T, double dt,
+                       double fugScale, const double *scaling, const double *wellRowScale,
+                       int32_t nwells, const nn_well *wells, const int64_t *perfCells,
+                       const double *perfIndex, const double *surfaceDensity,
+                       const double *primary, const double *phaseState,
+                       const double *accumPrev, double *residual, double *bsr,
+                       uint8_t *decisions, int threads)
+{
+    const Handle &h = *static_cast<Handle *>(hv);
+    Topology t{ncell, connOffsets, connNeighbor, connTrans, connGdz, rowOffsets,
+               colBlocks, volume, porosity, T};
+    const WellSet ws{nwells, wells, perfCells, perfIndex, surfaceDensity};
+    try
+    {
+        return dispatchNc(h.desc.nc, [&](auto nc) {
+            constexpr int NC = decltype(nc)::value;
+            return hasWell
+                ? assemblyRun<NC, true>(h, t, dt, fugScale, scaling, wellRowScale, primary,
+                                        phaseState, accumPrev, residual, bsr, decisions,
+                                        nullptr, threads, ws)
+                : assemblyRun<NC, false>(h, t, dt, fugScale, scaling, wellRowScale, primary,
+                                         phaseState, accumPrev, residual, bsr, decisions,
+                                         nullptr, threads, ws);
         });
     }
     catch (const std::exception &e)

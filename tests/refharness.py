@@ -43,6 +43,11 @@ def lib():
         L.ref_water_closures.argtypes = [_I64] + [_P] * 7
         L.ref_newton_limit.argtypes = [_P, _I64, ctypes.c_int, _P, _P, _P, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int]
+        L.ref_assembly_wells.argtypes = [_P, _I64, ctypes.c_int, _P, _P, _P, _P, _P, _P, _P, _P,
+                                         _P, ctypes.c_double, ctypes.c_double, _P, _P,
+                                         ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                         ctypes.c_int]
+        L.ref_last_error.restype = ctypes.c_char_p
         L.ref_csv_samples.argtypes = [_I64, _P, ctypes.c_char_p, _I64]
         L.ref_csv_samples.restype = _I64
         _lib = L
@@ -159,6 +164,42 @@ class Reference:
         self.L.ref_phase_update(self.h, n, int(has_well), _p(T), _p(primary), _p(phase_state),
                                 _p(status), _p(flags), int(threads))
         return primary, phase_state, status, flags
+
+    def assembly_wells(self, topo, volume, porosity, T, primary, phase_state, accum_prev, dt,
+                       wells, perforation_cells, perforation_well_index, surface_density,
+                       fug_scale=1.0, threads=1, scaling=None, well_row_scale=None):
+        """The restated runtime loops with wells (formFunction's perforation sources and
+        control rows, insertWellJacobian_), around the reference's own
+        computePerforationWellSourceInto.  wells: an nn_well ctypes array; the
+        pattern in `topo` must hold the wells' couplings.  Returns (residual, bsr)."""
+        n = len(T)
+        np_ = primary.shape[1]
+        arr = {k: np.ascontiguousarray(v) for k, v in topo.items()}
+        vol = np.ascontiguousarray(volume, np.float64)
+        por = np.ascontiguousarray(porosity, np.float64)
+        T = np.ascontiguousarray(T, np.float64)
+        primary = np.ascontiguousarray(primary, np.float64)
+        phase_state = np.ascontiguousarray(phase_state, np.float64)
+        accum_prev = np.ascontiguousarray(accum_prev, np.float64)
+        pc = np.ascontiguousarray(perforation_cells, np.int64)
+        wi = np.ascontiguousarray(perforation_well_index, np.float64)
+        sd = np.ascontiguousarray(surface_density, np.float64)
+        nnzb = int(arr["row_offsets"][-1])
+        res = np.zeros((n, np_))
+        bsr = np.zeros((nnzb, np_, np_))
+        dec = np.zeros((int(arr["conn_offsets"][-1]), 3), np.uint8)
+        sc = None if scaling is None else np.ascontiguousarray(scaling, np.float64)
+        wr = None if well_row_scale is None else np.ascontiguousarray(well_row_scale, np.float64)
+        rc = self.L.ref_assembly_wells(
+            self.h, n, 1, _p(arr["conn_offsets"]), _p(arr["conn_neighbor"]),
+            _p(arr["conn_trans"]), _p(arr["conn_gdz"]), _p(arr["row_offsets"]),
+            _p(arr["col_blocks"]), _p(vol), _p(por), _p(T), float(dt), float(fug_scale), _p(sc),
+            _p(wr), len(wells), ctypes.cast(wells, ctypes.c_void_p), _p(pc), _p(wi), _p(sd),
+            _p(primary), _p(phase_state), _p(accum_prev), _p(res), _p(bsr), _p(dec),
+            int(threads))
+        if rc != 0:
+            raise RuntimeError(f"ref_assembly_wells failed: {rc}: {self.L.ref_last_error()}")
+        return res, bsr
 
     def assembly(self, topo, volume, porosity, T, primary, phase_state, accum_prev, dt,
                  fug_scale=1.0, has_well=False, want_residual=True, want_bsr=True, threads=1,
