@@ -484,10 +484,13 @@ def learned_workload(args) -> dict:
         cmp["threshold"] = th
         results[name] = cmp
         log(f"hints {name}: {cmp}")
-    best = results["order"]   # the label-safe mode (bit-identical results)
+    # the order-only hint leaves each state's arithmetic unchanged; what it changes
+    # is which states share a wavefront, which the six-component device build's
+    # results depend on (label_mismatches of results["order"] count it)
+    best = results["order"]
     n = args.states
     return {
-        "metric": "learned-order batched flashes/s (FP64, bit-identical to the unhinted flash)",
+        "metric": "learned-order batched flashes/s (FP64, order-only hints)",
         "value": n / best["seeded_s"], "unit": "flashes/s", "n_gpus": 1,
         "steps": max(1, args.steps), "warmup": 1, "ms_per_step": best["seeded_s"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
