@@ -86,10 +86,11 @@ FP64_PEAK_TFLOPS = 78.6
 # profiles/phase_update_cpa_fp64_ops_single_cell_sample.md
 PHASE_UPDATE_FP64_OPS_PER_CELL = 5.9956e6
 # HBM bytes (L2 <-> memory, FETCH_SIZE x 2 + WRITE_SIZE x 1 against a 2 GiB copy)
-# of one headline step's kernels, from the two PMC passes of
-# profiles/phase_update_natural_assembly_200x200x50_cpa_hbm_traffic.md
+# of one headline step's kernels, from the PMC passes of
+# profiles/phase_update_natural_assembly_200x200x50_cpa_hbm_traffic.md (the
+# assembly kernel's from its `--workload assembly_only` passes)
 PHASE_UPDATE_TRAFFIC_BYTES = 2.3692e12
-ASSEMBLE_TRAFFIC_BYTES = 2.0598e11
+ASSEMBLE_TRAFFIC_BYTES = 1.3880e11
 
 
 def cpu_model() -> str:

@@ -576,15 +576,25 @@ __global__ void __launch_bounds__(256, kWavesPerEu) cellRecordKernel(
             accum[c * NC + k] = err.bad() ? nan : componentAccumulation<NC, S>(pr, k).v;
 }
 
+// Derivative chunks of the assembly: chunk k evaluates a cell with Dual<CH> seeded
+// on primary variables [k CH, k CH + CH).
+template <int NC, bool Well>
+constexpr int assemblyChunkCount()
+{
+    return (Layout<NC, Well>::NP + kAssemblyChunk - 1) / kAssemblyChunk;
+}
+
 // Residual + Jacobian columns of one cell (residual.inc:59-120, jacobian.inc:20-237),
-// in derivative chunks: chunk k evaluates the cell with Dual<CH> seeded on primary
-// variables [k CH, k CH + CH).  A derivative component is computed by the same
-// operations on the same values whatever the other components hold, so every
-// column is bit-identical to a Dual<N_p> evaluation, while a lane keeps CH + 1
-// doubles per AD quantity instead of N_p + 1 (private segment and spills shrink
-// with it).  The first chunk also computes the residual row and the upwind
-// decisions; the PR-CPA density roots with their correction slopes come from the
-// record pass (or, without it, from the first chunk) and are reused by all chunks.
+// one work-item per (cell, derivative chunk), the chunk index fastest: a
+// derivative component is computed by the same operations on the same values
+// whatever the other components hold, so every column is bit-identical to a
+// Dual<N_p> evaluation, while a lane keeps CH + 1 doubles per AD quantity instead
+// of N_p + 1.  Neighbouring lanes hold neighbouring column pairs of the same
+// cell, so each block row is stored by adjacent lanes of one store instruction.
+// Chunk 0 also computes the residual row, the upwind decisions, the well sources
+// and the status; the PR-CPA density roots with their correction slopes come from
+// the record pass and are reused by every chunk.  Without a Jacobian (bsr null)
+// the launch has one work-item per cell.
 template <int NC, bool Well>
 __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
     const FluidParams *__restrict__ fp, AssemblyView v, double dt, double fugScale,
@@ -597,7 +607,10 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
     constexpr int CH = kAssemblyChunk;
     constexpr int NCHUNK = (NP + CH - 1) / CH;
     using S = Dual<CH>;
-    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int nchunk = bsr ? NCHUNK : 1;
+    const int64_t item = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t c = item / nchunk;
+    const int chunk = int(item - c * nchunk);
     if (c >= v.ncell)
         return;
     const FluidParams &f = *fp;
@@ -620,7 +633,6 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
     if (v.roots)
         for (int k = 0; k < 9; ++k) roots[k] = v.roots[k * v.ncell + c];
     int32_t wellStatus = NN_STATUS_OK;   // a perforation source the reference rejects
-    for (int chunk = 0; chunk < NCHUNK; ++chunk)
     {
         const int col0 = chunk * CH;
         Err err;
@@ -635,20 +647,24 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
         }
         if (err.bad())
         {
-            // values are those of the first chunk, so only the first chunk gets here
+            // the values are the same in every chunk, so every chunk of the cell
+            // gets here: chunk 0 writes the status and the residual, each chunk
+            // its columns of the cell's blocks
             const double nan = __builtin_nan("");
-            if (status) status[c] = err.code;
-            if (residual)
-                for (int r = 0; r < NP; ++r) residual[c * NP + r] = nan;
-            if (bsr)
+            if (chunk == 0)
             {
-                for (int e = 0; e < NP * NP; ++e) self[e] = nan;
-                for (int64_t k = kBegin; k < kEnd; ++k)
-                {
-                    double *rb = bsr + v.recipSlot[k] * NP * NP;
-                    for (int e = 0; e < NP * NP; ++e) rb[e] = nan;
-                }
+                if (status) status[c] = err.code;
+                if (residual)
+                    for (int r = 0; r < NP; ++r) residual[c * NP + r] = nan;
             }
+            if (bsr)
+                for (int q = 0; q < CH && col0 + q < NP; ++q)
+                    for (int r = 0; r < NP; ++r)
+                    {
+                        self[r * NP + col0 + q] = nan;
+                        for (int64_t k = kBegin; k < kEnd; ++k)
+                            bsr[v.recipSlot[k] * NP * NP + r * NP + col0 + q] = nan;
+                    }
             return;
         }
         S res[NP];
@@ -747,9 +763,8 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
                     rb[i * NP + col] = -flux[i].d[q] * v.rowScale[i] * v.varScale[col];
                 }
             }
-            if (chunk == 0)
-                for (int r = NC; r < NP; ++r)
-                    for (int q = 0; q < NP; ++q) rb[r * NP + q] = 0.0;
+            for (int q = 0; q < CH && col0 + q < NP; ++q)
+                for (int r = NC; r < NP; ++r) rb[r * NP + col0 + q] = 0.0;
         }
         if (self)
             for (int q = 0; q < CH; ++q)
@@ -792,10 +807,8 @@ __global__ void __launch_bounds__(64, kWavesPerEu) assembleKernel(
             for (int r = NC; r < NP; ++r)
                 residual[c * NP + r] = res[r].v * rowScale[r];
         }
-        if (!self)
-            break;
     }
-    if (status)
+    if (status && chunk == 0)
         status[c] = wellStatus;
 }
 
@@ -1065,7 +1078,9 @@ hipError_t Launch<NC>::assemble(bool well, hipStream_t st, const FluidParams *fp
     }
     else
     {
-    const unsigned grid = static_cast<unsigned>((v.ncell + 63) / 64);
+    const int chunks = well ? assemblyChunkCount<NC, true>() : assemblyChunkCount<NC, false>();
+    const int64_t items = v.ncell * (bsr ? chunks : 1);
+    const unsigned grid = static_cast<unsigned>((items + 63) / 64);
     if (well)
         assembleKernel<NC, true><<<grid, 64, 0, st>>>(fp, v, dt, fugScale, primary, phaseState,
                                                       accumPrev, residual, bsr, decisions, status);

@@ -230,10 +230,10 @@ static int run(const FluidParams &fp, const FlashOptions &op, FILE *in, FILE *ou
         cellRecordKernel<NC, Well>(&fp, v, rPrim.data(), rPs.data(), nullptr, astat.data());
     }
     emuBlockDim.x = 64;
-    for (int64_t c = 0; c < n; ++c)
+    for (int64_t t = 0; t < n * assemblyChunkCount<NC, Well>(); ++t)
     {
-        emuBlockIdx.x = unsigned(c / 64);
-        emuThreadIdx.x = unsigned(c % 64);
+        emuBlockIdx.x = unsigned(t / 64);
+        emuThreadIdx.x = unsigned(t % 64);
         assembleKernel<NC, Well>(&fp, v, dt, 1.0, rPrim.data(), rPs.data(), rAcc.data(),
                                  residual.data(), bsr.data(), dec.data(), astat.data());
     }
