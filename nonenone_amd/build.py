@@ -37,6 +37,9 @@ if WAVES_PER_EU not in (None, "", "1"):
                        "faulted on the GPU with a memory-aperture violation)")
 # Derivative columns per chunk of the assembly kernel's AD (csrc/launch.hpp).
 ASSEMBLY_CHUNK = os.environ.get("NN_ASSEMBLY_CHUNK")
+# Rachford-Rice recovery work after which a lane hands its state to the
+# wavefront-per-state pass (csrc/common.hpp).
+DEFER_BUDGET = os.environ.get("NN_DEFER_BUDGET")
 ARCH = os.environ.get("NN_OFFLOAD_ARCH", "gfx950")
 UNITS = ("nn_api.hip", "nn_nc2.hip", "nn_nc3.hip", "nn_nc4.hip", "nn_nc5.hip", "nn_nc6.hip", "nn_nc8.hip",
          "mlp.hip", "serve.hip")
@@ -63,6 +66,8 @@ def _compile(unit: str) -> tuple[str, Path, str]:
     extra = [f"-DNN_WAVES_PER_EU={int(WAVES_PER_EU)}"] if WAVES_PER_EU else []
     if ASSEMBLY_CHUNK:
         extra.append(f"-DNN_ASSEMBLY_CHUNK={int(ASSEMBLY_CHUNK)}")
+    if DEFER_BUDGET:
+        extra.append(f"-DNN_DEFER_BUDGET={int(DEFER_BUDGET)}")
     cmd = [hipcc(), *FLAGS, *extra, "-Rpass-analysis=kernel-resource-usage",
            "-I", str(HERE.parent / "include"), "-c", str(CSRC / unit), "-o", str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)

@@ -95,7 +95,10 @@ enum : int
     kModeDefer = 1,
     kModeCoop = 2,
 };
-constexpr int kDeferBudget = 20000;  // RR3 residual evaluations per state
+#ifndef NN_DEFER_BUDGET
+#define NN_DEFER_BUDGET 20000
+#endif
+constexpr int kDeferBudget = NN_DEFER_BUDGET;  // RR3 residual evaluations per state
 
 // Wavefront helpers for kModeCoop.  The host build sees single-lane stand-ins,
 // or, with NN_HOST_WAVE_EMULATION (test tooling: tests/coop_emulation.cpp), a
