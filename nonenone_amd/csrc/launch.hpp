@@ -33,8 +33,9 @@ constexpr int kBlock = 64;  // one wavefront per block: divergent per-state cont
 #endif
 constexpr int kWavesPerEu = NN_WAVES_PER_EU;
 
-// Derivative columns per chunk of the assembly kernel's forward-mode AD: a lane
-// evaluates its cell once per chunk of primary variables with Dual<kAssemblyChunk>.
+// Derivative columns per chunk of the assembly kernel's forward-mode AD: each
+// (cell, chunk of primary variables) pair is one work-item evaluating the cell
+// with Dual<kAssemblyChunk>.
 // Build parameter (nonenone_amd/build.py, NN_ASSEMBLY_CHUNK).
 #ifndef NN_ASSEMBLY_CHUNK
 #define NN_ASSEMBLY_CHUNK 2
