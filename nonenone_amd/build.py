@@ -40,6 +40,9 @@ ASSEMBLY_CHUNK = os.environ.get("NN_ASSEMBLY_CHUNK")
 # Rachford-Rice recovery work after which a lane hands its state to the
 # wavefront-per-state pass (csrc/common.hpp).
 DEFER_BUDGET = os.environ.get("NN_DEFER_BUDGET")
+# 0: the lane pass runs the SW allocation recovery itself instead of handing the
+# state on (csrc/common.hpp).
+DEFER_ALLOCATION = os.environ.get("NN_DEFER_ALLOCATION")
 ARCH = os.environ.get("NN_OFFLOAD_ARCH", "gfx950")
 UNITS = ("nn_api.hip", "nn_nc2.hip", "nn_nc3.hip", "nn_nc4.hip", "nn_nc5.hip", "nn_nc6.hip", "nn_nc8.hip",
          "mlp.hip", "serve.hip")
@@ -68,6 +71,8 @@ def _compile(unit: str) -> tuple[str, Path, str]:
         extra.append(f"-DNN_ASSEMBLY_CHUNK={int(ASSEMBLY_CHUNK)}")
     if DEFER_BUDGET:
         extra.append(f"-DNN_DEFER_BUDGET={int(DEFER_BUDGET)}")
+    if DEFER_ALLOCATION:
+        extra.append(f"-DNN_DEFER_ALLOCATION={int(DEFER_ALLOCATION)}")
     cmd = [hipcc(), *FLAGS, *extra, "-Rpass-analysis=kernel-resource-usage",
            "-I", str(HERE.parent / "include"), "-c", str(CSRC / unit), "-o", str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)

@@ -82,8 +82,9 @@ enum : int
 // Execution modes of the flash code.
 //   kModePlain: one work-item per state, everything serial in that work-item.
 //   kModeDefer: as plain, but a state whose Rachford-Rice recovery work passes
-//               kDeferBudget, or that enters the SW allocation recovery, stops
-//               with kErrDefer so the wave-cooperative pass can redo it.
+//               kDeferBudget, or (with kDeferAllocation) that enters the SW
+//               allocation recovery, stops with kErrDefer so the wave-cooperative
+//               pass can redo it.
 //   kModeCoop:  one wavefront per state; every lane runs the state's serial
 //               control flow on identical data, and the independent candidates
 //               of the recovery searches (RR3 starting guesses, allocation
@@ -99,6 +100,12 @@ enum : int
 #define NN_DEFER_BUDGET 20000
 #endif
 constexpr int kDeferBudget = NN_DEFER_BUDGET;  // RR3 residual evaluations per state
+// Whether kModeDefer hands a state entering the SW allocation recovery to the
+// wavefront-per-state pass (1) or runs the recovery's seeds serially on its lane (0).
+#ifndef NN_DEFER_ALLOCATION
+#define NN_DEFER_ALLOCATION 1
+#endif
+constexpr bool kDeferAllocation = NN_DEFER_ALLOCATION != 0;
 
 // Wavefront helpers for kModeCoop.  The host build sees single-lane stand-ins,
 // or, with NN_HOST_WAVE_EMULATION (test tooling: tests/coop_emulation.cpp), a

@@ -554,49 +554,72 @@ public:
     }
 
     // nonlinear.inc:8-93
-    __host__ __device__ Rr3 solveRr3(const double *z, const double *kg, const double *kw,
-                                     double ibg, double ibw) const
+    // Start of one RR3 Newton solve from (ibg, ibw): clamped into the interior.
+    __host__ __device__ static void rr3Start(double ibg, double ibw, double &bg, double &bw)
     {
-        double bg = clampd(ibg, 1.0e-4, 0.8);
-        double bw = clampd(ibw, 1.0e-4, 0.8);
+        bg = clampd(ibg, 1.0e-4, 0.8);
+        bw = clampd(ibw, 1.0e-4, 0.8);
         if (bg + bw >= 0.95)
         {
             const double scale = 0.8 / (bg + bw);
             bg *= scale;
             bw *= scale;
         }
+    }
+
+    // One iteration of the RR3 Newton solve with its backtracking line search:
+    // kRr3Running (moved to a point of smaller residual norm), kRr3Converged
+    // (the norm at (bg, bw) is below the tolerance) or kRr3Failed.
+    enum : int
+    {
+        kRr3Running = 0,
+        kRr3Converged = 1,
+        kRr3Failed = 2,
+    };
+    __host__ __device__ int rr3Step(const double *z, const double *kg, const double *kw,
+                                    double &bg, double &bw) const
+    {
+        double fg, fw, jgg, jww, jgw;
+        ++work;
+        const double norm = rr3Norm(z, kg, kw, bg, bw, &fg, &fw, &jgg, &jww, &jgw);
+        if (!finite(norm))
+            return kRr3Failed;
+        if (norm < o.rrTol)
+            return kRr3Converged;
+        const double det = jgg * jww - jgw * jgw;
+        if (::fabs(det) < 1.0e-24 || !finite(det))
+            return kRr3Failed;
+        const double dbg = (-fg * jww + fw * jgw) / det;
+        const double dbw = (-fw * jgg + fg * jgw) / det;
+        double lambda = 1.0;
+        for (int ls = 0; ls < 40; ++ls)
+        {
+            const double cg = bg + lambda * dbg;
+            const double cw = bw + lambda * dbw;
+            ++work;
+            const double cn = rr3Norm(z, kg, kw, cg, cw, nullptr, nullptr, nullptr, nullptr, nullptr);
+            if (finite(cn) && cn < norm)
+            {
+                bg = cg;
+                bw = cw;
+                return kRr3Running;
+            }
+            lambda *= 0.5;
+        }
+        return kRr3Failed;
+    }
+
+    __host__ __device__ Rr3 solveRr3(const double *z, const double *kg, const double *kw,
+                                     double ibg, double ibw) const
+    {
+        double bg, bw;
+        rr3Start(ibg, ibw, bg, bw);
         for (int it = 0; it < o.maxRrIterations; ++it)
         {
-            double fg, fw, jgg, jww, jgw;
-            ++work;
-            const double norm = rr3Norm(z, kg, kw, bg, bw, &fg, &fw, &jgg, &jww, &jgw);
-            if (!finite(norm))
-                return {false, 0.0, 0.0};
-            if (norm < o.rrTol)
+            const int r = rr3Step(z, kg, kw, bg, bw);
+            if (r == kRr3Converged)
                 return {true, bg, bw};
-            const double det = jgg * jww - jgw * jgw;
-            if (::fabs(det) < 1.0e-24 || !finite(det))
-                return {false, 0.0, 0.0};
-            const double dbg = (-fg * jww + fw * jgw) / det;
-            const double dbw = (-fw * jgg + fg * jgw) / det;
-            double lambda = 1.0;
-            bool accepted = false;
-            for (int ls = 0; ls < 40; ++ls)
-            {
-                const double cg = bg + lambda * dbg;
-                const double cw = bw + lambda * dbw;
-                ++work;
-                const double cn = rr3Norm(z, kg, kw, cg, cw, nullptr, nullptr, nullptr, nullptr, nullptr);
-                if (finite(cn) && cn < norm)
-                {
-                    bg = cg;
-                    bw = cw;
-                    accepted = true;
-                    break;
-                }
-                lambda *= 0.5;
-            }
-            if (!accepted)
+            if (r == kRr3Failed)
                 return {false, 0.0, 0.0};
         }
         return {false, 0.0, 0.0};
@@ -613,15 +636,33 @@ public:
             {0.01, 0.90}, {0.45, 0.45}};
         if constexpr (Mode == kModeCoop)
         {
-            // Guess g on lane g; the lowest converged lane is the serial loop's answer.
+            // Guess g on lane g, all lanes one Newton iteration at a time; the
+            // lowest converged lane is the serial loop's answer, and the wave stops
+            // as soon as every lane below it has failed (the serial loop would not
+            // have tried the guesses above it).
             const int lane = waveLane();
-            Rr3 r{false, 0.0, 0.0};
+            double bg = 0.0, bw = 0.0;
+            int state = kRr3Failed;
             if (lane < 17)
-                r = solveRr3(z, kg, kw, guesses[lane][0], guesses[lane][1]);
-            const int first = firstSetLane(waveBallot(r.converged));
+            {
+                rr3Start(guesses[lane][0], guesses[lane][1], bg, bw);
+                state = kRr3Running;
+            }
+            for (int it = 0; it < o.maxRrIterations; ++it)
+            {
+                if (state == kRr3Running)
+                    state = rr3Step(z, kg, kw, bg, bw);
+                const unsigned long long running = waveBallot(state == kRr3Running);
+                const int first = firstSetLane(waveBallot(state == kRr3Converged));
+                if (running == 0ull ||
+                    (first >= 0 && (running & ((1ull << first) - 1ull)) == 0ull))
+                    break;
+            }
+            // a lane still running here has used the serial solve's iterations: failed
+            const int first = firstSetLane(waveBallot(state == kRr3Converged));
             if (first < 0)
                 return {false, 0.0, 0.0};
-            return {true, waveShfl(r.bg, first), waveShfl(r.bw, first)};
+            return {true, waveShfl(bg, first), waveShfl(bw, first)};
         }
         else
         {
@@ -3175,7 +3216,7 @@ public:
     {
         Result best;
         double bestG = __builtin_inf();
-        if constexpr (Mode == kModeDefer)
+        if constexpr (Mode == kModeDefer && kDeferAllocation)
         {
             err.raise(kErrDefer);
             return best;
